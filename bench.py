@@ -1,0 +1,204 @@
+"""H|psi> throughput on one MI355X: the 64-atom cell, FP64 vs mixed.
+
+A step is one H|psi> application to all 256 bands of the 64-atom cell
+(Si64 diamond 2x2x2 supercell, a = 20.52 bohr, ecut 40 Ry, npw 36,945,
+90^3 FFT grid, 512 projectors, k = (0.1, 0, 0)), with psi already resident
+in HBM.  Prints one JSON line (rank 0) with throughput, the roofline of the
+dominant kernel (HIP events on the library's stream), the mixed-vs-FP64
+energy error, FP64 parity against the CPU oracle and the CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+RY_PER_HA = 2.0
+
+
+def kernel_model(name, prec, nb, npw, nplanes, plane, N, nch):
+    """Algorithmic bytes (or flops) of one launch of a kernel class over nb
+    bands: each declared input / output counted once at storage precision."""
+    S = 16 if prec == "fp64" else 8
+    R = 8 if prec == "fp64" else 4
+    if name == "plane_g2r":        # psi gather + (ig, pos) int32 pairs, plane writes
+        return "hbm", nb * (npw * (S + 8) + nplanes * plane * S)
+    if name == "xpencil_vloc":     # plane read + write, V_loc once
+        return "hbm", nb * 2 * nplanes * plane * S + N * R
+    if name == "plane_r2g":        # plane read, psi read, hpsi write, indices, kinetic
+        return "hbm", nb * (nplanes * plane * S + npw * (2 * S + 8)) + npw * 8
+    if name == "proj_gemm":        # <p|psi>: 8 flop per complex MAC
+        return "mfma", 8.0 * nch * npw * nb
+    if name == "proj_apply":
+        return "mfma", 8.0 * nch * npw * nb
+    if name == "proj_reduce":
+        return "hbm", None
+    return "hbm", None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-bands", type=int, default=32, help="bands in the 1-core CPU sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--timing-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the H|psi> path shards over "
+                                   "k-points and bands without collectives (see DESIGN.md), "
+                                   "multi-GPU runs are not part of it"}))
+        sys.exit(2)
+
+    import torch
+    from effmass_amd import synth
+    from effmass_amd.hamiltonian import Hamiltonian
+
+    t_setup = time.time()
+    wl = synth.build(1)
+    cell, kp, psi_h = wl.cell, wl.kpts[0], wl.psi[0]
+    nb, npw = psi_h.shape
+    H = Hamiltonian(cell, device=0)
+    H.load_kpoint(0, kp)
+    dev = torch.device("cuda:0")
+    psi64 = torch.from_numpy(psi_h).to(dev)
+    psi32 = psi64.to(torch.complex64)
+    h64 = torch.zeros_like(psi64)
+    h32 = torch.zeros_like(psi32)
+    op = H.at_k(0)
+    setup_s = time.time() - t_setup
+
+    def run(prec, steps):
+        p, h = (psi64, h64) if prec == "fp64" else (psi32, h32)
+        for _ in range(steps):
+            op(p, h, prec=prec, overwrite=True)
+
+    res = {}
+    for prec in ("fp64", "mixed"):
+        run(prec, args.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(prec, args.steps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[prec] = {"ms_per_step": dt * 1e3 / args.steps, "bands_per_s": nb * args.steps / dt}
+
+    # energies: FP64 Rayleigh quotients of both paths
+    op(psi64, h64, prec="fp64", overwrite=True)
+    op(psi32, h32, prec="mixed", overwrite=True)
+    E64 = H.rayleigh(psi64, h64).cpu().numpy()
+    E32 = H.rayleigh(psi32, h32, prec="mixed").cpu().numpy()
+    dE_ry = float(np.abs(E32 - E64).max() * RY_PER_HA)
+
+    # per-kernel durations with HIP events on the library's stream
+    geo = dict(nb=nb, npw=npw, nplanes=len(np.unique(kp.gidx[:, 0] % cell.n[0])),
+               plane=cell.n[1] * cell.n[2], N=int(np.prod(cell.n)), nch=cell.nchannels())
+    kernels = []
+    for prec in ("fp64", "mixed"):
+        H.reset_timing()
+        H.set_timing(True)
+        run(prec, args.timing_steps)
+        torch.cuda.synchronize()
+        for name in ("plane_g2r", "xpencil_vloc", "plane_r2g", "proj_gemm", "proj_reduce", "proj_apply"):
+            ms, cnt = H.timing(name)
+            if cnt == 0:
+                continue
+            avg_s = ms / cnt * 1e-3
+            kind = "fp64" if prec == "fp64" else "fp32"
+            bound, amount = kernel_model(name, "fp64" if prec == "fp64" else "fp32", **geo)
+            ent = {"kernel": name, "prec": prec, "avg_us": avg_s * 1e6, "launches": cnt,
+                   "share_of_step": ms / cnt / res[prec]["ms_per_step"], "bound": bound}
+            if amount is not None:
+                if bound == "hbm":
+                    ach = amount / avg_s / 1e9
+                    ent.update(achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
+                               algorithmic_bytes=amount)
+                else:
+                    ach = amount / avg_s / 1e12
+                    pk = MFMA_PEAK_TFLOPS[kind]
+                    ent.update(achieved=ach, peak=pk, unit="TFLOP/s", frac=ach / pk, algorithmic_flops=amount)
+            kernels.append(ent)
+        H.set_timing(False)
+    dom = max((k for k in kernels if k["prec"] == "mixed" and "frac" in k), key=lambda k: k["share_of_step"])
+    roofline = {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
+                "frac": dom["frac"], "traffic": None, "kernel": dom["kernel"], "prec": "mixed"}
+
+    # CPU baseline: the FP64 oracle (reference loop structure) on host cores
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        nsb = min(args.cpu_bands, nb)
+        sample = psi_h[:nsb]
+        oracle.hpsi(cell, kp, sample[:1])                      # warm-up
+        t0 = time.perf_counter()
+        ref = oracle.hpsi(cell, kp, sample)
+        t1 = time.perf_counter() - t0
+        nthr = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        oracle.hpsi(cell, kp, psi_h[:4 * nthr], nthreads=nthr)
+        tn = time.perf_counter() - t0
+        cpu = {"value": nsb / t1, "unit": "bands/s", "cores": 1, "kind": "port",
+               "sample": f"{nsb} bands of the 64-atom cell, FP64, 1 thread, {t1:.1f} s",
+               "value_all_cores": 4 * nthr / tn, "cores_all": nthr,
+               "cpu_model": _cpu_model()}
+        op(psi64, h64, prec="fp64", overwrite=True)
+        got = h64[:nsb].cpu().numpy()
+        rel = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        op(psi32, h32, prec="mixed", overwrite=True)
+        got32 = h32[:nsb].cpu().numpy()
+        rel32 = np.linalg.norm(got32 - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        parity = {"fp64_max_rel_err_vs_oracle": float(rel.max()),
+                  "mixed_max_rel_err_vs_oracle": float(rel32.max()), "bands_checked": nsb}
+
+    out = {
+        "metric": "Hψ bands/s, 64-atom cell, mixed vs FP64 at 1e-6 Ry; p50/p99 m* query latency",
+        "value": res["mixed"]["bands_per_s"],
+        "unit": "bands/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["mixed"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 FFT/GEMM, fp64 D contraction + Rayleigh",
+        "data": "synthetic (seeded SplitMix64, effmass_amd/synth.py config 1)",
+        "config": {"workload": "Si64 2x2x2 diamond supercell, a=20.52 bohr, ecut 40 Ry, k=(0.1,0,0), "
+                               "npw 36945, FFT 90^3, 256 bands, 512 projectors; step = H|psi> on all bands",
+                   "bands": nb, "npw": npw, "grid": list(cell.n), "projectors": cell.nchannels()},
+        "fp64_bands_per_s": res["fp64"]["bands_per_s"],
+        "fp64_ms_per_step": res["fp64"]["ms_per_step"],
+        "mixed_over_fp64": res["mixed"]["bands_per_s"] / res["fp64"]["bands_per_s"],
+        "energy_err_mixed_vs_fp64_ry": dE_ry,
+        "parity": parity,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+        "setup_s": setup_s,
+    }
+    print(json.dumps(out), flush=True)
+    H.close()
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+if __name__ == "__main__":
+    main()

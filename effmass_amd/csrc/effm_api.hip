@@ -1,0 +1,717 @@
+// effm_api.hip — host side of the C-ABI declared in include/effm_hpsi.h.
+//
+// Owns the device-resident cell (V_loc, beta tables, channels, FFT plans)
+// and k-point data (G-sphere, plane index, projector matrices), and
+// sequences the kernels of k_local.hip / k_nonlocal.hip on one stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "effm_hpsi.h"
+#include "effm_internal.hpp"
+
+struct effm_ctx {
+    effm::Ctx c;
+};
+
+namespace effm {
+
+// ---------------------------------------------------------------------------
+//  small helpers
+// ---------------------------------------------------------------------------
+static int fail(Ctx& c, int code, const std::string& msg) {
+    c.err = msg;
+    return code;
+}
+
+static int hip_fail(Ctx& c, hipError_t e, const char* where) {
+    c.err = std::string(where) + ": " + hipGetErrorString(e);
+    if (e == hipErrorOutOfMemory) return kErrNoMem;
+    return kErrHip;
+}
+
+#define HIP_TRY(expr, where)                          \
+    do {                                              \
+        hipError_t _e = (expr);                       \
+        if (_e != hipSuccess) return hip_fail(c, _e, where); \
+    } while (0)
+
+static void dfree(Ctx& c, DevBuf& b) {
+    if (b.ptr) {
+        (void)hipFree(b.ptr);
+        c.used -= b.bytes;
+    }
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+static int dalloc(Ctx& c, DevBuf& b, size_t bytes, const char* what) {
+    dfree(c, b);
+    if (bytes == 0) return kOk;
+    if (c.budget && c.used + bytes > c.budget)
+        return fail(c, kErrNoMem, std::string("HBM budget exceeded allocating ") + what + " (" +
+                                      std::to_string(bytes) + " B, in use " + std::to_string(c.used) +
+                                      " B, budget " + std::to_string(c.budget) + " B)");
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        return fail(c, kErrNoMem, std::string("hipMalloc failed for ") + what + ": " + hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    c.used += bytes;
+    return kOk;
+}
+
+static int ensure(Ctx& c, DevBuf& b, size_t bytes, const char* what) {
+    if (b.bytes >= bytes && b.ptr) return kOk;
+    return dalloc(c, b, bytes, what);
+}
+
+template <typename V>
+static int upload(Ctx& c, DevBuf& b, const std::vector<V>& h, const char* what) {
+    int rc = dalloc(c, b, h.size() * sizeof(V), what);
+    if (rc) return rc;
+    if (!h.empty()) HIP_TRY(hipMemcpy(b.ptr, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice), what);
+    return kOk;
+}
+
+// Radix sequence for an axis: 4s, then 2, 3, 5, then remaining primes.
+static bool factor_axis(int n, FftAxis& ax) {
+    ax.n = n;
+    ax.ns = 0;
+    int m = n;
+    const int pref[] = {4, 2, 3, 5};
+    for (int p : pref)
+        while (m % p == 0 && m > 1) {
+            if (ax.ns >= kMaxStages) return false;
+            ax.p[ax.ns++] = p;
+            m /= p;
+        }
+    const int more[] = {7, 11, 13};
+    for (int p : more)
+        while (m % p == 0) {
+            if (ax.ns >= kMaxStages) return false;
+            ax.p[ax.ns++] = p;
+            m /= p;
+        }
+    return m == 1;
+}
+
+// perm(j): position of input j for the in-place DIT (= position of output
+// j for the DIF), the mixed-radix digit reversal of fft_lds.hpp.
+static std::vector<int> fft_perm(const FftAxis& ax) {
+    std::vector<int> L(ax.ns + 1, 1);
+    for (int t = 0; t < ax.ns; ++t) L[t + 1] = L[t] * ax.p[t];
+    std::vector<int> P(ax.n);
+    for (int j = 0; j < ax.n; ++j) {
+        int jj = j, pos = 0;
+        for (int t = ax.ns - 1; t >= 0; --t) {
+            int d = jj % ax.p[t];
+            jj /= ax.p[t];
+            pos += d * L[t];
+        }
+        P[j] = pos;
+    }
+    return P;
+}
+
+static void twiddles(int n, std::vector<double2>& w64, std::vector<float2>& w32) {
+    w64.resize(n);
+    w32.resize(n);
+    for (int k = 0; k < n; ++k) {
+        long double a = 2.0L * (long double)k / (long double)n;   // angle / pi
+        long double pi = 3.141592653589793238462643383279502884L;
+        double cr = (double)cosl(pi * a), ci = (double)(-sinl(pi * a));
+        w64[k] = double2{cr, ci};
+        w32[k] = float2{(float)cr, (float)ci};
+    }
+}
+
+void timer_begin(Ctx& c) {
+    if (!c.timing) return;
+    hipEvent_t e;
+    if (!c.event_pool.empty()) {
+        e = c.event_pool.back();
+        c.event_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        c.cur_start = nullptr;
+        return;
+    }
+    (void)hipEventRecord(e, c.stream);
+    c.cur_start = e;
+}
+
+void timer_end(Ctx& c, const char* name) {
+    if (!c.timing || !c.cur_start) return;
+    hipEvent_t e;
+    if (!c.event_pool.empty()) {
+        e = c.event_pool.back();
+        c.event_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        return;
+    }
+    (void)hipEventRecord(e, c.stream);
+    c.pending.push_back({name, c.cur_start, e});
+    c.cur_start = nullptr;
+}
+
+static void timer_resolve(Ctx& c) {
+    for (auto& p : c.pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            auto& t = c.timers[p.name];
+            t.ms += ms;
+            t.count += 1;
+        }
+        c.event_pool.push_back(p.a);
+        c.event_pool.push_back(p.b);
+    }
+    c.pending.clear();
+}
+
+static void release_kpt(Ctx& c, KptDev& k) {
+    for (DevBuf* b : {&k.K, &k.kin, &k.gidx, &k.plane_x, &k.x_to_plane, &k.plane_ptr, &k.ent_ig,
+                      &k.ent_pos, &k.P64, &k.P32, &k.dP64, &k.dP32})
+        dfree(c, *b);
+    k = KptDev{};
+}
+
+static void release_cell(Ctx& c) {
+    for (auto& k : c.kpts) release_kpt(c, k);
+    c.kpts.clear();
+    for (DevBuf* b : {&c.vloc64, &c.vloc32, &c.tw1_64, &c.tw2_64, &c.tw3_64, &c.tw1_32, &c.tw2_32,
+                      &c.tw3_32, &c.perm1, &c.tau, &c.btab, &c.dbtab, &c.tab_info, &c.chan_dev,
+                      &c.lam64})
+        dfree(c, *b);
+    c.chans.clear();
+    c.tinfo.clear();
+    c.have_cell = false;
+}
+
+static int check_kpt(Ctx& c, int ikpt) {
+    if (!c.have_cell) return fail(c, kErrRuntime, "H|psi> requires a cell (effm_cell_upload)");
+    if (ikpt < 0 || ikpt >= (int)c.kpts.size() || !c.kpts[ikpt].valid)
+        return fail(c, kErrOutOfRange, "Invalid k-point index: " + std::to_string(ikpt));
+    return kOk;
+}
+
+// Bytes of FFT plane workspace per band.
+template <typename T>
+static size_t work_per_band(const Ctx& c, const KptDev& k) {
+    return (size_t)k.nplanes * c.n2 * c.n3 * sizeof(cx_t<T>);
+}
+
+static constexpr size_t kWorkCap = size_t(3) << 30;   // FFT workspace cap (bytes)
+
+template <typename T>
+static int ensure_proj(Ctx& c, KptDev& k, bool grad) {
+    using C = cx_t<T>;
+    if (c.nch() == 0) return kOk;
+    const bool f64 = sizeof(T) == 8;
+    DevBuf& P = f64 ? k.P64 : k.P32;
+    DevBuf& dP = f64 ? k.dP64 : k.dP32;
+    const size_t pbytes = (size_t)c.nch() * k.npw * sizeof(C);
+    bool need = !P.ptr || (grad && !dP.ptr);
+    if (!need) return kOk;
+    if (grad && !c.have_dbeta)
+        return fail(c, kErrRuntime, "dH/dk requires dbeta/dq tables for every species");
+    int rc;
+    if (!P.ptr && (rc = dalloc(c, P, pbytes, "projectors"))) return rc;
+    if (grad && !dP.ptr && (rc = dalloc(c, dP, 3 * pbytes, "projector k-gradients"))) return rc;
+    HIP_TRY(launch_build_proj<T>(c, k, grad), "build projectors");
+    return kOk;
+}
+
+template <typename T>
+static int run_hpsi(Ctx& c, KptDev& k, int nb, const void* psi, int64_t ld, void* hpsi,
+                    int64_t ldo, bool acc) {
+    using C = cx_t<T>;
+    int rc = ensure_proj<T>(c, k, false);
+    if (rc) return rc;
+    const size_t wpb = work_per_band<T>(c, k);
+    int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)nb, kWorkCap / std::max<size_t>(wpb, 1)));
+    if ((rc = ensure(c, c.work, wpb * chunk, "FFT workspace"))) return rc;
+    for (int b0 = 0; b0 < nb; b0 += chunk) {
+        int nbc = std::min(chunk, nb - b0);
+        HIP_TRY(launch_local<T>(c, k, nbc, static_cast<const C*>(psi) + (int64_t)b0 * ld, ld,
+                                static_cast<C*>(hpsi) + (int64_t)b0 * ldo, ldo, acc, c.work.ptr),
+                "local potential kernels");
+    }
+    if (c.nch() > 0) {
+        size_t pb = vnl_partial_bytes(c.nch(), nb, k.npw, sizeof(C));
+        if ((rc = ensure(c, c.partial, pb, "projection partial sums"))) return rc;
+        if ((rc = ensure(c, c.cbuf, (size_t)c.nch() * nb * sizeof(C), "projections"))) return rc;
+        HIP_TRY(launch_vnl<T>(c, k, nb, psi, ld, hpsi, ldo, c.partial.ptr, c.partial.bytes, c.cbuf.ptr),
+                "nonlocal kernels");
+    }
+    return kOk;
+}
+
+template <typename T>
+static int run_dhdk(Ctx& c, KptDev& k, int nb, const void* psi, int64_t ld, void* ox, void* oy,
+                    void* oz, int64_t ldo, bool acc) {
+    using C = cx_t<T>;
+    int rc = ensure_proj<T>(c, k, true);
+    if (rc) return rc;
+    if (c.nch() > 0) {
+        size_t pb = vnl_partial_bytes(4 * c.nch(), nb, k.npw, sizeof(C));
+        if ((rc = ensure(c, c.partial, pb, "projection partial sums"))) return rc;
+        if ((rc = ensure(c, c.cbuf, (size_t)4 * c.nch() * nb * sizeof(C), "projections"))) return rc;
+    }
+    HIP_TRY(launch_dhdk<T>(c, k, nb, psi, ld, ox, oy, oz, ldo, acc, c.partial.ptr, c.partial.bytes,
+                           c.cbuf.ptr),
+            "dH/dk kernels");
+    return kOk;
+}
+
+}  // namespace effm
+
+using namespace effm;
+
+// ===========================================================================
+//  C-ABI
+// ===========================================================================
+extern "C" {
+
+int effm_ctx_create(effm_ctx** out, int device, uint64_t hbm_budget) {
+    if (!out) return kErrInvalid;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return kErrHip;
+    if (device < 0 || device >= ndev) return kErrOutOfRange;
+    if (hipSetDevice(device) != hipSuccess) return kErrHip;
+    auto* h = new (std::nothrow) effm_ctx;
+    if (!h) return kErrNoMem;
+    h->c.device = device;
+    h->c.budget = hbm_budget;
+    h->c.stream = nullptr;   // legacy default stream until effm_set_stream
+    *out = h;
+    return kOk;
+}
+
+int effm_ctx_destroy(effm_ctx* h) {
+    if (!h) return kOk;
+    Ctx& c = h->c;
+    (void)hipSetDevice(c.device);
+    (void)hipStreamSynchronize(c.stream);
+    release_cell(c);
+    for (DevBuf* b : {&c.work, &c.partial, &c.cbuf, &c.stage_in, &c.stage_out0, &c.stage_out1, &c.stage_out2})
+        dfree(c, *b);
+    timer_resolve(c);
+    for (auto e : c.event_pool) (void)hipEventDestroy(e);
+    delete h;
+    return kOk;
+}
+
+int effm_set_stream(effm_ctx* h, void* stream) {
+    if (!h) return kErrInvalid;
+    h->c.stream = static_cast<hipStream_t>(stream);
+    return kOk;
+}
+
+const char* effm_last_error(const effm_ctx* h) {
+    if (!h) return "null context";
+    return h->c.err.c_str();
+}
+
+int effm_mem_info(const effm_ctx* h, uint64_t* used, uint64_t* budget) {
+    if (!h) return kErrInvalid;
+    if (used) *used = h->c.used;
+    if (budget) *budget = h->c.budget;
+    return kOk;
+}
+
+int effm_cell_upload(effm_ctx* h, const effm_cell* cell) {
+    if (!h || !cell) return kErrInvalid;
+    Ctx& c = h->c;
+    (void)hipSetDevice(c.device);
+    if (cell->n1 <= 0 || cell->n2 <= 0 || cell->n3 <= 0)
+        return fail(c, kErrInvalid, "FFT grid dimensions must be positive");
+    if (!cell->vloc) return fail(c, kErrRuntime, "H|psi> requires VR (vloc is NULL)");
+    if (cell->natom < 0 || (cell->natom > 0 && (!cell->species || !cell->tau)))
+        return fail(c, kErrRuntime, "H|psi> requires ATOM (species / tau missing)");
+    FftAxis a1, a2, a3;
+    if (!factor_axis(cell->n1, a1) || !factor_axis(cell->n2, a2) || !factor_axis(cell->n3, a3))
+        return fail(c, kErrInvalid, "FFT grid sizes must factor into 2, 3, 5, 7, 11, 13");
+    size_t plane64 = sizeof(double2) * ((size_t)cell->n2 * cell->n3 + cell->n2 + cell->n3);
+    if (plane64 > 160 * 1024)
+        return fail(c, kErrInvalid, "n2*n3 plane exceeds the 160 KiB LDS of a gfx950 CU");
+    release_cell(c);
+    c.n1 = cell->n1; c.n2 = cell->n2; c.n3 = cell->n3;
+    std::memcpy(c.A, cell->lattice, sizeof(c.A));
+    const double* A = cell->lattice;
+    double det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+                 A[2] * (A[3] * A[7] - A[4] * A[6]);
+    c.omega = std::fabs(det);
+    if (!(c.omega > 0)) return fail(c, kErrInvalid, "lattice is singular");
+    c.ax1 = a1; c.ax2 = a2; c.ax3 = a3;
+    c.hperm1 = fft_perm(a1); c.hperm2 = fft_perm(a2); c.hperm3 = fft_perm(a3);
+    int rc;
+    size_t N = (size_t)c.n1 * c.n2 * c.n3;
+    std::vector<double> v64(cell->vloc, cell->vloc + N);
+    std::vector<float> v32(N);
+    for (size_t i = 0; i < N; ++i) v32[i] = (float)v64[i];
+    if ((rc = upload(c, c.vloc64, v64, "vloc"))) return rc;
+    if ((rc = upload(c, c.vloc32, v32, "vloc fp32"))) return rc;
+    std::vector<double2> w64;
+    std::vector<float2> w32;
+    twiddles(c.n1, w64, w32);
+    if ((rc = upload(c, c.tw1_64, w64, "twiddles")) || (rc = upload(c, c.tw1_32, w32, "twiddles"))) return rc;
+    twiddles(c.n2, w64, w32);
+    if ((rc = upload(c, c.tw2_64, w64, "twiddles")) || (rc = upload(c, c.tw2_32, w32, "twiddles"))) return rc;
+    twiddles(c.n3, w64, w32);
+    if ((rc = upload(c, c.tw3_64, w64, "twiddles")) || (rc = upload(c, c.tw3_32, w32, "twiddles"))) return rc;
+    if ((rc = upload(c, c.perm1, c.hperm1, "perm"))) return rc;
+
+    // atoms, tables, channels (reference order: species -> atom -> (l, ib) -> m)
+    c.natom = cell->natom;
+    std::vector<double> tau(cell->tau, cell->tau + 3 * (size_t)cell->natom);
+    if ((rc = upload(c, c.tau, tau, "tau"))) return rc;
+    const double norm = 1.0 / std::sqrt(c.omega);
+    int nq_max = 0;
+    bool all_dbeta = true;
+    for (int s = 0; s < cell->nspecies; ++s) {
+        const effm_species& sp = cell->sp[s];
+        if (sp.lmax < 0 || sp.nbeta <= 0) continue;
+        nq_max = std::max(nq_max, (int)sp.nq);
+        if (!sp.dbeta_tab) all_dbeta = false;
+    }
+    std::vector<int> tab_of(0);
+    std::vector<double> btab, dbtab;
+    std::vector<TableDev> tdev;
+    c.lmax = -1;
+    c.tables_qmax = 1e300;
+    std::vector<std::vector<int>> sp_table(cell->nspecies);
+    for (int s = 0; s < cell->nspecies; ++s) {
+        const effm_species& sp = cell->sp[s];
+        if (sp.lmax < 0 || sp.nbeta <= 0) continue;
+        if (sp.nq < 4 || !(sp.dq > 0) || !sp.beta_tab || !sp.beta_l || !sp.lambda)
+            return fail(c, kErrInvalid, "species " + std::to_string(s) + ": incomplete beta tables");
+        if (sp.lmax > 6) return fail(c, kErrInvalid, "lmax > 6 is not supported");
+        c.lmax = std::max(c.lmax, (int)sp.lmax);
+        for (int ib = 0; ib < sp.nbeta; ++ib) {
+            if (ib > 0 && sp.beta_l[ib] < sp.beta_l[ib - 1])
+                return fail(c, kErrInvalid, "beta channels must be sorted by l (sortByL)");
+            if (sp.beta_l[ib] < 0 || sp.beta_l[ib] > sp.lmax)
+                return fail(c, kErrInvalid, "beta_l out of [0, lmax]");
+            int row = (int)tdev.size();
+            sp_table[s].push_back(row);
+            btab.resize((size_t)(row + 1) * nq_max, 0.0);
+            std::copy(sp.beta_tab + (size_t)ib * sp.nq, sp.beta_tab + (size_t)(ib + 1) * sp.nq,
+                      btab.begin() + (size_t)row * nq_max);
+            TableDev t{};
+            t.nq = sp.nq;
+            t.dq = sp.dq;
+            t.qmax = sp.qmax;
+            t.norm = norm;
+            t.dbeta0 = 0.0;
+            if (all_dbeta) {
+                dbtab.resize((size_t)(row + 1) * nq_max, 0.0);
+                std::copy(sp.dbeta_tab + (size_t)ib * sp.nq, sp.dbeta_tab + (size_t)(ib + 1) * sp.nq,
+                          dbtab.begin() + (size_t)row * nq_max);
+                t.dbeta0 = sp.dbeta_tab[(size_t)ib * sp.nq] * norm;   // interpolate(q = 0)
+            }
+            // largest q the 4-point stencil can serve: i0 + 3 < nq
+            double qcap = std::min(sp.qmax, sp.dq * (double)(sp.nq - 3) - 1e-12 * sp.dq);
+            c.tables_qmax = std::min(c.tables_qmax, qcap);
+            tdev.push_back(t);
+        }
+    }
+    c.have_dbeta = all_dbeta;
+    c.nq_max = nq_max;
+    // channels
+    std::vector<ChanDev> chd;
+    std::vector<double> lam;
+    for (int s = 0; s < cell->nspecies; ++s) {
+        const effm_species& sp = cell->sp[s];
+        if (sp.lmax < 0 || sp.nbeta <= 0) continue;
+        for (int a = 0; a < cell->natom; ++a) {
+            if (cell->species[a] != s) continue;
+            for (int ib = 0; ib < sp.nbeta; ++ib) {
+                double l_ = sp.lambda[ib];
+                if (std::fabs(l_) < 1e-15) continue;   // callable.cpp:227
+                int l = sp.beta_l[ib];
+                for (int m = -l; m <= l; ++m) {
+                    c.chans.push_back({a, sp_table[s][ib], l, m, l_, s});
+                    chd.push_back({a, sp_table[s][ib], l, m});
+                    lam.push_back(l_);
+                }
+            }
+        }
+    }
+    for (int a = 0; a < cell->natom; ++a)
+        if (cell->species[a] < 0 || cell->species[a] >= cell->nspecies)
+            return fail(c, kErrOutOfRange, "atom species index out of range");
+    if (!tdev.empty()) {
+        if ((rc = upload(c, c.btab, btab, "beta tables"))) return rc;
+        if (all_dbeta && (rc = upload(c, c.dbtab, dbtab, "dbeta tables"))) return rc;
+        if ((rc = upload(c, c.tab_info, tdev, "table info"))) return rc;
+    }
+    if (!chd.empty()) {
+        if ((rc = upload(c, c.chan_dev, chd, "channels"))) return rc;
+        if ((rc = upload(c, c.lam64, lam, "lambda"))) return rc;
+    }
+    c.have_cell = true;
+    return kOk;
+}
+
+int effm_kpt_upload(effm_ctx* h, int ikpt, int npw, const double* K, const double* kinetic,
+                    const int32_t* gidx, const double kfrac[3]) {
+    if (!h) return kErrInvalid;
+    Ctx& c = h->c;
+    (void)hipSetDevice(c.device);
+    if (!c.have_cell) return fail(c, kErrRuntime, "effm_kpt_upload requires a cell");
+    if (ikpt < 0) return fail(c, kErrOutOfRange, "Invalid k-point index: " + std::to_string(ikpt));
+    if (npw <= 0 || !K || !kinetic || !gidx)
+        return fail(c, kErrRuntime, "current k-point has no G-vectors");
+    // GKK::validateKineticConsistency (src/io/GKK.cppm:652-671)
+    double qmax_seen = 0.0;
+    for (int g = 0; g < npw; ++g) {
+        double kx = K[3 * g], ky = K[3 * g + 1], kz = K[3 * g + 2];
+        double e = 0.5 * (kx * kx + ky * ky + kz * kz);
+        double d = kinetic[g] - e;
+        if (d > 1e-10 * (1.0 + e) || d < -1e-10 * (1.0 + e))
+            return fail(c, kErrRuntime, "GKK kinetic-energy mismatch at k-point " + std::to_string(ikpt) +
+                                            ", G-vector " + std::to_string(g));
+        qmax_seen = std::max(qmax_seen, std::sqrt(kx * kx + ky * ky + kz * kz));
+    }
+    // BetaqTables::interpolate throws for q > q_max (ncpp_advance.cppm:659-661)
+    if (c.nch() > 0 && qmax_seen > c.tables_qmax)
+        return fail(c, kErrDomain, "BetaqTables::interpolate: q exceeds q_max");
+    const int n1 = c.n1, n2 = c.n2, n3 = c.n3;
+    std::vector<int> xi(npw), pos(npw);
+    std::vector<int> plane_of_x(n1, -1);
+    std::vector<char> seen((size_t)n1 * n2 * n3, 0);
+    for (int g = 0; g < npw; ++g) {
+        int i = ((gidx[3 * g] % n1) + n1) % n1;
+        int j = ((gidx[3 * g + 1] % n2) + n2) % n2;
+        int k = ((gidx[3 * g + 2] % n3) + n3) % n3;
+        size_t lin = ((size_t)i * n2 + j) * n3 + k;
+        if (seen[lin])
+            return fail(c, kErrInvalid, "two G-vectors map to one FFT grid point (grid too small)");
+        seen[lin] = 1;
+        xi[g] = i;
+        pos[g] = c.hperm2[j] * n3 + c.hperm3[k];
+        plane_of_x[i] = 1;
+    }
+    std::vector<int> plane_x;
+    for (int i = 0; i < n1; ++i)
+        if (plane_of_x[i] >= 0) {
+            plane_of_x[i] = (int)plane_x.size();
+            plane_x.push_back(i);
+        }
+    const int np = (int)plane_x.size();
+    std::vector<int> ptr(np + 1, 0);
+    for (int g = 0; g < npw; ++g) ptr[plane_of_x[xi[g]] + 1]++;
+    for (int p = 0; p < np; ++p) ptr[p + 1] += ptr[p];
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1), ent_ig(npw), ent_pos(npw);
+    for (int g = 0; g < npw; ++g) {
+        int p = plane_of_x[xi[g]];
+        int e = fill[p]++;
+        ent_ig[e] = g;
+        ent_pos[e] = pos[g];
+    }
+    if (ikpt >= (int)c.kpts.size()) c.kpts.resize(ikpt + 1);
+    KptDev& k = c.kpts[ikpt];
+    release_kpt(c, k);
+    k.npw = npw;
+    for (int d = 0; d < 3; ++d) k.kfrac[d] = kfrac ? kfrac[d] : 0.0;
+    k.qmax_seen = qmax_seen;
+    std::vector<double> Ks((size_t)3 * npw), kin(kinetic, kinetic + npw);
+    for (int g = 0; g < npw; ++g)
+        for (int d = 0; d < 3; ++d) Ks[(size_t)d * npw + g] = K[3 * g + d];
+    std::vector<int> gi(gidx, gidx + 3 * (size_t)npw);
+    int rc;
+    if ((rc = upload(c, k.K, Ks, "K"))) return rc;
+    if ((rc = upload(c, k.kin, kin, "kinetic"))) return rc;
+    if ((rc = upload(c, k.gidx, gi, "gidx"))) return rc;
+    if ((rc = upload(c, k.plane_x, plane_x, "plane list"))) return rc;
+    if ((rc = upload(c, k.x_to_plane, plane_of_x, "plane map"))) return rc;
+    if ((rc = upload(c, k.plane_ptr, ptr, "plane ptr"))) return rc;
+    if ((rc = upload(c, k.ent_ig, ent_ig, "plane entries"))) return rc;
+    if ((rc = upload(c, k.ent_pos, ent_pos, "plane positions"))) return rc;
+    k.nplanes = np;
+    int mx = 0;
+    for (int p = 0; p < np; ++p) mx = std::max(mx, ptr[p + 1] - ptr[p]);
+    k.max_plane_entries = mx;
+    k.valid = true;
+    return kOk;
+}
+
+int effm_kpt_dim(const effm_ctx* h, int ikpt) {
+    if (!h) return kErrInvalid;
+    const Ctx& c = h->c;
+    if (ikpt < 0 || ikpt >= (int)c.kpts.size() || !c.kpts[ikpt].valid) return kErrOutOfRange;
+    return c.kpts[ikpt].npw;
+}
+
+int effm_kpt_release(effm_ctx* h, int ikpt) {
+    if (!h) return kErrInvalid;
+    Ctx& c = h->c;
+    if (ikpt < 0 || ikpt >= (int)c.kpts.size()) return fail(c, kErrOutOfRange, "Invalid k-point index");
+    (void)hipStreamSynchronize(c.stream);
+    release_kpt(c, c.kpts[ikpt]);
+    return kOk;
+}
+
+static int stage_copy(Ctx& c, DevBuf& buf, const void* host, size_t bytes, bool copy, const char* what) {
+    int rc = ensure(c, buf, bytes, what);
+    if (rc) return rc;
+    if (copy && host) HIP_TRY(hipMemcpyAsync(buf.ptr, host, bytes, hipMemcpyHostToDevice, c.stream), what);
+    return kOk;
+}
+
+int effm_hpsi(effm_ctx* h, int ikpt, int nb, const void* psi, int64_t ld, void* hpsi,
+              int64_t ld_out, int prec, int flags) {
+    if (!h) return kErrInvalid;
+    Ctx& c = h->c;
+    (void)hipSetDevice(c.device);
+    int rc = check_kpt(c, ikpt);
+    if (rc) return rc;
+    KptDev& k = c.kpts[ikpt];
+    if (nb < 0) return fail(c, kErrInvalid, "nb must be non-negative");
+    if (nb == 0) return kOk;
+    if (!psi || !hpsi) return fail(c, kErrInvalid, "psi / hpsi must not be NULL");
+    if (ld < k.npw || ld_out < k.npw) return fail(c, kErrInvalid, "leading dimension smaller than npw");
+    if (prec != EFFM_PREC_FP64 && prec != EFFM_PREC_MIXED) return fail(c, kErrInvalid, "unknown precision");
+    const bool acc = !(flags & EFFM_F_OVERWRITE);
+    const size_t esz = prec == EFFM_PREC_FP64 ? 16 : 8;
+    const void* dpsi = psi;
+    void* dh = hpsi;
+    if (flags & EFFM_F_HOST) {
+        if ((rc = stage_copy(c, c.stage_in, psi, esz * ld * nb, true, "staging psi"))) return rc;
+        if ((rc = stage_copy(c, c.stage_out0, hpsi, esz * ld_out * nb, acc, "staging hpsi"))) return rc;
+        dpsi = c.stage_in.ptr;
+        dh = c.stage_out0.ptr;
+    }
+    rc = prec == EFFM_PREC_FP64 ? run_hpsi<double>(c, k, nb, dpsi, ld, dh, ld_out, acc)
+                                : run_hpsi<float>(c, k, nb, dpsi, ld, dh, ld_out, acc);
+    if (rc) return rc;
+    if (flags & EFFM_F_HOST) {
+        HIP_TRY(hipMemcpyAsync(hpsi, dh, esz * ld_out * nb, hipMemcpyDeviceToHost, c.stream), "copy hpsi");
+        HIP_TRY(hipStreamSynchronize(c.stream), "sync");
+    }
+    return kOk;
+}
+
+int effm_dhdk(effm_ctx* h, int ikpt, int nb, const void* psi, int64_t ld, void* ox, void* oy,
+              void* oz, int64_t ld_out, int prec, int flags) {
+    if (!h) return kErrInvalid;
+    Ctx& c = h->c;
+    (void)hipSetDevice(c.device);
+    int rc = check_kpt(c, ikpt);
+    if (rc) return rc;
+    KptDev& k = c.kpts[ikpt];
+    if (nb < 0) return fail(c, kErrInvalid, "nb must be non-negative");
+    if (nb == 0 || (!ox && !oy && !oz)) return kOk;
+    if (!psi) return fail(c, kErrInvalid, "psi must not be NULL");
+    if (ld < k.npw || ld_out < k.npw) return fail(c, kErrInvalid, "leading dimension smaller than npw");
+    if (prec != EFFM_PREC_FP64 && prec != EFFM_PREC_MIXED) return fail(c, kErrInvalid, "unknown precision");
+    const bool acc = !(flags & EFFM_F_OVERWRITE);
+    const size_t esz = prec == EFFM_PREC_FP64 ? 16 : 8;
+    const void* dpsi = psi;
+    void* outs[3] = {ox, oy, oz};
+    void* douts[3] = {ox, oy, oz};
+    DevBuf* st[3] = {&c.stage_out0, &c.stage_out1, &c.stage_out2};
+    if (flags & EFFM_F_HOST) {
+        if ((rc = stage_copy(c, c.stage_in, psi, esz * ld * nb, true, "staging psi"))) return rc;
+        dpsi = c.stage_in.ptr;
+        for (int a = 0; a < 3; ++a) {
+            if (!outs[a]) continue;
+            if ((rc = stage_copy(c, *st[a], outs[a], esz * ld_out * nb, acc, "staging dH/dk"))) return rc;
+            douts[a] = st[a]->ptr;
+        }
+    }
+    rc = prec == EFFM_PREC_FP64 ? run_dhdk<double>(c, k, nb, dpsi, ld, douts[0], douts[1], douts[2], ld_out, acc)
+                                : run_dhdk<float>(c, k, nb, dpsi, ld, douts[0], douts[1], douts[2], ld_out, acc);
+    if (rc) return rc;
+    if (flags & EFFM_F_HOST) {
+        for (int a = 0; a < 3; ++a)
+            if (outs[a])
+                HIP_TRY(hipMemcpyAsync(outs[a], douts[a], esz * ld_out * nb, hipMemcpyDeviceToHost, c.stream),
+                        "copy dH/dk");
+        HIP_TRY(hipStreamSynchronize(c.stream), "sync");
+    }
+    return kOk;
+}
+
+int effm_hpsi_multi(effm_ctx* h, int nk, const int* ikpts, int nb, const void* const* psi_ptrs,
+                    const int64_t* ld, void* const* hpsi_ptrs, int prec, int flags) {
+    if (!h) return kErrInvalid;
+    Ctx& c = h->c;
+    if (nk < 0 || (nk > 0 && (!ikpts || !psi_ptrs || !ld || !hpsi_ptrs)))
+        return fail(c, kErrInvalid, "invalid batch description");
+    if (flags & EFFM_F_HOST) return fail(c, kErrInvalid, "effm_hpsi_multi takes device pointers only");
+    for (int i = 0; i < nk; ++i) {
+        int rc = effm_hpsi(h, ikpts[i], nb, psi_ptrs[i], ld[i], hpsi_ptrs[i], ld[i], prec, flags);
+        if (rc) return rc;
+    }
+    return kOk;
+}
+
+int effm_rayleigh(effm_ctx* h, int npw, int nb, const void* psi, int64_t ld, const void* hpsi,
+                  int64_t ld_h, int prec, double* E, int flags) {
+    if (!h) return kErrInvalid;
+    Ctx& c = h->c;
+    (void)hipSetDevice(c.device);
+    if (nb <= 0) return kOk;
+    if (!psi || !hpsi || !E || npw <= 0 || ld < npw || ld_h < npw)
+        return fail(c, kErrInvalid, "invalid Rayleigh-quotient arguments");
+    const size_t esz = prec == EFFM_PREC_FP64 ? 16 : 8;
+    const void* dp = psi;
+    const void* dh = hpsi;
+    double* dE = E;
+    int rc;
+    if (flags & EFFM_F_HOST) {
+        if ((rc = stage_copy(c, c.stage_in, psi, esz * ld * nb, true, "staging psi"))) return rc;
+        if ((rc = stage_copy(c, c.stage_out0, hpsi, esz * ld_h * nb, true, "staging hpsi"))) return rc;
+        if ((rc = ensure(c, c.stage_out1, sizeof(double) * nb, "staging E"))) return rc;
+        dp = c.stage_in.ptr;
+        dh = c.stage_out0.ptr;
+        dE = static_cast<double*>(c.stage_out1.ptr);
+    }
+    timer_begin(c);
+    hipError_t e = prec == EFFM_PREC_FP64 ? launch_rayleigh<double>(c.stream, npw, nb, dp, ld, dh, ld_h, dE)
+                                          : launch_rayleigh<float>(c.stream, npw, nb, dp, ld, dh, ld_h, dE);
+    timer_end(c, "rayleigh");
+    if (e != hipSuccess) return hip_fail(c, e, "rayleigh");
+    if (flags & EFFM_F_HOST) {
+        HIP_TRY(hipMemcpyAsync(E, dE, sizeof(double) * nb, hipMemcpyDeviceToHost, c.stream), "copy E");
+        HIP_TRY(hipStreamSynchronize(c.stream), "sync");
+    }
+    return kOk;
+}
+
+int effm_set_timing(effm_ctx* h, int enable) {
+    if (!h) return kErrInvalid;
+    h->c.timing = enable != 0;
+    return kOk;
+}
+
+int effm_get_timing(const effm_ctx* hc, const char* name, double* ms, int64_t* count) {
+    if (!hc || !name) return kErrInvalid;
+    effm_ctx* h = const_cast<effm_ctx*>(hc);
+    timer_resolve(h->c);
+    auto it = h->c.timers.find(name);
+    if (it == h->c.timers.end()) {
+        if (ms) *ms = 0.0;
+        if (count) *count = 0;
+        return kOk;
+    }
+    if (ms) *ms = it->second.ms;
+    if (count) *count = it->second.count;
+    return kOk;
+}
+
+int effm_reset_timing(effm_ctx* h) {
+    if (!h) return kErrInvalid;
+    timer_resolve(h->c);
+    h->c.timers.clear();
+    return kOk;
+}
+
+}  // extern "C"

@@ -1,0 +1,179 @@
+// effm_internal.hpp — context layout and kernel launchers of libeffm_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fft_lds.hpp"
+
+namespace effm {
+
+// Status codes (mirrors include/effm_hpsi.h).
+enum : int {
+    kOk = 0,
+    kErrRuntime = -1,
+    kErrOutOfRange = -2,
+    kErrDomain = -3,
+    kErrInvalid = -4,
+    kErrNoMem = -5,
+    kErrHip = -6,
+};
+
+struct Status {
+    int code = kOk;
+    std::string msg;
+    static Status ok() { return {}; }
+    bool good() const { return code == kOk; }
+};
+
+// Device buffer owned by the context (counted against the HBM budget).
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+// Projector channel: one (atom, l, ib, m) with |lambda| >= 1e-15, in the
+// reference's loop order type -> atom -> (l, ib) -> m
+// (src/H_psi/hamiltonian_callable.cpp:211-261).
+struct Channel {
+    int atom;
+    int table;      // row of the concatenated beta / dbeta tables
+    int l;
+    int m;
+    double lambda;
+    int species;
+};
+
+struct TableInfo {   // per table row
+    int nq;
+    double dq;
+    double qmax;
+    double dbeta0;   // dbeta/dq(0) * 1/sqrt(Omega), the q->0 limit of beta/q
+};
+
+struct KptDev {
+    bool valid = false;
+    int npw = 0;
+    double kfrac[3] = {0, 0, 0};
+    double qmax_seen = 0;
+    // SoA K (3*npw), kinetic (npw), gidx (3*npw)
+    DevBuf K, kin, gidx;
+    // plane CSR: active x-planes, entries grouped by plane
+    int nplanes = 0;
+    DevBuf plane_x;      // int[nplanes]
+    DevBuf x_to_plane;   // int[n1], -1 if inactive
+    DevBuf plane_ptr;    // int[nplanes+1]
+    DevBuf ent_ig;       // int[npw]
+    DevBuf ent_pos;      // int[npw] digit-reversed (j,k) position in the plane
+    int max_plane_entries = 0;
+    // projector matrices, built on first use: P[c][g]; dP[alpha][c][g]
+    DevBuf P64, P32, dP64, dP32;
+};
+
+struct Timer {
+    double ms = 0.0;
+    int64_t count = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    std::string err;
+
+    // cell
+    bool have_cell = false;
+    int n1 = 0, n2 = 0, n3 = 0;
+    double A[9] = {0};
+    double omega = 0;
+    int natom = 0;
+    int lmax = -1;
+    FftAxis ax1{}, ax2{}, ax3{};
+    DevBuf vloc64, vloc32;
+    DevBuf tw1_64, tw2_64, tw3_64, tw1_32, tw2_32, tw3_32;
+    DevBuf perm1;                       // int[n1]
+    std::vector<int> hperm1, hperm2, hperm3;
+    DevBuf tau;                         // double[natom*3]
+    std::vector<Channel> chans;
+    std::vector<TableInfo> tinfo;
+    double tables_qmax = 0;             // min qmax over tables
+    DevBuf btab, dbtab;                 // double[ntab*nq]
+    DevBuf tab_info;                    // TableDev[ntab]
+    DevBuf chan_dev;                    // ChanDev[nch]
+    DevBuf lam64;                       // double[nch]
+    bool have_dbeta = false;
+    int nq_max = 0;
+
+    std::vector<KptDev> kpts;
+
+    // scratch (grown on demand)
+    DevBuf work;       // FFT plane workspace
+    DevBuf partial;    // split-K partial sums
+    DevBuf cbuf;       // reduced <p|psi> (scaled), precision of the path
+    DevBuf stage_in, stage_out0, stage_out1, stage_out2;   // host-pointer staging
+
+    bool timing = false;
+    std::map<std::string, Timer> timers;
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    hipEvent_t cur_start = nullptr;
+
+    int nch() const { return static_cast<int>(chans.size()); }
+};
+
+// ---- device-side descriptors -------------------------------------------
+struct ChanDev {
+    int atom;
+    int table;
+    int l;
+    int m;
+};
+
+struct TableDev {
+    int nq;
+    int pad;
+    double dq;
+    double qmax;
+    double norm;     // 1/sqrt(Omega)
+    double dbeta0;   // normalised dbeta/dq at q = 0
+};
+
+// ---- launchers (k_local.hip) --------------------------------------------
+template <typename T>
+hipError_t launch_local(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld,
+                        void* hpsi, int64_t ldo, bool accumulate, void* work);
+
+// ---- launchers (k_nonlocal.hip) -----------------------------------------
+template <typename T>
+hipError_t launch_build_proj(Ctx& c, KptDev& k, bool grad);
+
+template <typename T>
+hipError_t launch_vnl(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld,
+                      void* hpsi, int64_t ldo, void* partial, size_t partial_bytes,
+                      void* cbuf);
+
+template <typename T>
+hipError_t launch_dhdk(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld,
+                       void* ox, void* oy, void* oz, int64_t ldo, bool accumulate,
+                       void* partial, size_t partial_bytes, void* cbuf);
+
+size_t vnl_partial_bytes(int nrows, int nb, int npw, size_t elem);
+
+template <typename T>
+hipError_t launch_rayleigh(hipStream_t s, int npw, int nb, const void* psi, int64_t ld,
+                           const void* hpsi, int64_t ldh, double* E);
+
+// timing helpers (effm_api.hip)
+void timer_begin(Ctx& c);
+void timer_end(Ctx& c, const char* name);
+
+}  // namespace effm

@@ -1,0 +1,201 @@
+// k_local.hip — kinetic + local-potential part of H|psi> on gfx950.
+//
+// Reference: src/H_psi/hamiltonian_callable.cpp:119-182 (kinetic, scatter,
+// G2R FFT, V_loc multiply, R2G FFT, gather) and src/math/fft.cppm:60-85.
+//
+// Three launches per band batch, each an HBM streaming pass:
+//   1. plane_g2r     one workgroup per (active x-plane, band): scatter the
+//                    plane's G-sphere entries into a zeroed n2 x n3 LDS
+//                    plane at digit-reversed positions, DIT backward FFT
+//                    along k then j, write the plane.
+//   2. xpencil_vloc  one workgroup per (j, z-chunk, band): load the x-pencils
+//                    of the active planes (inactive x = 0), backward FFT
+//                    along x, multiply by V_loc(r), forward FFT along x,
+//                    store only the active planes (the R2G result is needed
+//                    only inside the sphere).
+//   3. plane_r2g     one workgroup per (active x-plane, band): forward FFT
+//                    along j and k (DIF), then gather the sphere entries:
+//                    hpsi = [hpsi +] kin*psi + V psi / N.
+// Planes outside the sphere's x-range are never written or read: this is
+// the zero-padding the reference's dense 3D FFT spends on them.
+#include "effm_internal.hpp"
+
+namespace effm {
+
+template <typename T>
+__global__ void __launch_bounds__(512)
+k_plane_g2r(const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict__ work, int nplanes,
+            const int* __restrict__ plane_ptr, const int* __restrict__ ent_ig,
+            const int* __restrict__ ent_pos, FftAxis a2, FftAxis a3,
+            const cx_t<T>* __restrict__ tw2, const cx_t<T>* __restrict__ tw3) {
+    using C = cx_t<T>;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n2 = a2.n, n3 = a3.n, n23 = n2 * n3;
+    C* plane = reinterpret_cast<C*>(smem_raw);
+    C* t2 = plane + n23;
+    C* t3 = t2 + n2;
+    const int p = blockIdx.x;
+    const int b = blockIdx.y;
+    for (int i = threadIdx.x; i < n23; i += blockDim.x) plane[i] = C{0, 0};
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) t2[i] = tw2[i];
+    for (int i = threadIdx.x; i < n3; i += blockDim.x) t3[i] = tw3[i];
+    __syncthreads();
+    const C* ps = psi + (int64_t)b * ld;
+    const int e0 = plane_ptr[p], e1 = plane_ptr[p + 1];
+    for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) plane[ent_pos[e]] = ps[ent_ig[e]];
+    __syncthreads();
+    fft_lds<+1, true, T>(plane, a3, 1, n3, n2, t3);    // along k (rows)
+    fft_lds<+1, true, T>(plane, a2, n3, 1, n3, t2);    // along j (columns)
+    C* out = work + ((int64_t)b * nplanes + p) * n23;
+    for (int i = threadIdx.x; i < n23; i += blockDim.x) out[i] = plane[i];
+}
+
+template <typename T, int TZ>
+__global__ void __launch_bounds__(256)
+k_xpencil_vloc(cx_t<T>* __restrict__ work, int nplanes, const int* __restrict__ plane_x,
+               const int* __restrict__ x_to_plane, const int* __restrict__ perm1,
+               const T* __restrict__ vloc, FftAxis a1, int n2, int n3,
+               const cx_t<T>* __restrict__ tw1) {
+    using C = cx_t<T>;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n1 = a1.n;
+    C* buf = reinterpret_cast<C*>(smem_raw);      // [n1][TZ]
+    C* t1 = buf + n1 * TZ;
+    const int nzc = (n3 + TZ - 1) / TZ;
+    const int j = blockIdx.x / nzc;
+    const int z0 = (blockIdx.x - j * nzc) * TZ;
+    const int nz = min(TZ, n3 - z0);
+    const int b = blockIdx.y;
+    const int64_t n23 = (int64_t)n2 * n3;
+    cx_t<T>* wb = work + (int64_t)b * nplanes * n23 + (int64_t)j * n3 + z0;
+    for (int i = threadIdx.x; i < n1; i += blockDim.x) t1[i] = tw1[i];
+    for (int idx = threadIdx.x; idx < n1 * TZ; idx += blockDim.x) {
+        int x = idx / TZ, z = idx - x * TZ;
+        int pl = x_to_plane[x];
+        C v = C{0, 0};
+        if (pl >= 0 && z < nz) v = wb[(int64_t)pl * n23 + z];
+        buf[perm1[x] * TZ + z] = v;
+    }
+    __syncthreads();
+    fft_lds<+1, true, T>(buf, a1, TZ, 1, TZ, t1);      // G2R along x
+    const T* vb = vloc + (int64_t)j * n3 + z0;
+    for (int idx = threadIdx.x; idx < n1 * TZ; idx += blockDim.x) {
+        int x = idx / TZ, z = idx - x * TZ;
+        if (z < nz) {
+            T v = vb[(int64_t)x * n23 + z];
+            C a = buf[idx];
+            buf[idx] = C{a.x * v, a.y * v};
+        }
+    }
+    __syncthreads();
+    fft_lds<-1, false, T>(buf, a1, TZ, 1, TZ, t1);     // R2G along x
+    for (int idx = threadIdx.x; idx < nplanes * TZ; idx += blockDim.x) {
+        int pl = idx / TZ, z = idx - pl * TZ;
+        if (z < nz) wb[(int64_t)pl * n23 + z] = buf[perm1[plane_x[pl]] * TZ + z];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(512)
+k_plane_r2g(const cx_t<T>* __restrict__ work, int nplanes, const int* __restrict__ plane_ptr,
+            const int* __restrict__ ent_ig, const int* __restrict__ ent_pos,
+            const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict__ hpsi, int64_t ldo,
+            const double* __restrict__ kin, T scale, int accumulate, FftAxis a2, FftAxis a3,
+            const cx_t<T>* __restrict__ tw2, const cx_t<T>* __restrict__ tw3) {
+    using C = cx_t<T>;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n2 = a2.n, n3 = a3.n, n23 = n2 * n3;
+    C* plane = reinterpret_cast<C*>(smem_raw);
+    C* t2 = plane + n23;
+    C* t3 = t2 + n2;
+    const int p = blockIdx.x;
+    const int b = blockIdx.y;
+    const C* in = work + ((int64_t)b * nplanes + p) * n23;
+    for (int i = threadIdx.x; i < n23; i += blockDim.x) plane[i] = in[i];
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) t2[i] = tw2[i];
+    for (int i = threadIdx.x; i < n3; i += blockDim.x) t3[i] = tw3[i];
+    __syncthreads();
+    fft_lds<-1, false, T>(plane, a2, n3, 1, n3, t2);   // along j
+    fft_lds<-1, false, T>(plane, a3, 1, n3, n2, t3);   // along k
+    const C* ps = psi + (int64_t)b * ld;
+    C* hs = hpsi + (int64_t)b * ldo;
+    const int e0 = plane_ptr[p], e1 = plane_ptr[p + 1];
+    for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        int ig = ent_ig[e];
+        C v = plane[ent_pos[e]];
+        C s = ps[ig];
+        double kn = kin[ig];
+        // kinetic term in FP64 (callable.cpp:119-126), then the gathered
+        // local-potential term (callable.cpp:171-182)
+        double re = kn * (double)s.x + (double)(v.x * scale);
+        double im = kn * (double)s.y + (double)(v.y * scale);
+        if (accumulate) {
+            C h = hs[ig];
+            re += (double)h.x;
+            im += (double)h.y;
+        }
+        hs[ig] = C{(T)re, (T)im};
+    }
+}
+
+template <typename T>
+static size_t plane_lds_bytes(const Ctx& c) {
+    return sizeof(cx_t<T>) * ((size_t)c.n2 * c.n3 + c.n2 + c.n3);
+}
+
+template <typename T>
+hipError_t launch_local(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld, void* hpsi,
+                        int64_t ldo, bool accumulate, void* work) {
+    using C = cx_t<T>;
+    constexpr int TZ = 16;
+    const bool f64 = sizeof(T) == 8;
+    const C* tw1 = static_cast<const C*>(f64 ? c.tw1_64.ptr : c.tw1_32.ptr);
+    const C* tw2 = static_cast<const C*>(f64 ? c.tw2_64.ptr : c.tw2_32.ptr);
+    const C* tw3 = static_cast<const C*>(f64 ? c.tw3_64.ptr : c.tw3_32.ptr);
+    const T* vl = static_cast<const T*>(f64 ? c.vloc64.ptr : c.vloc32.ptr);
+    const size_t lds_plane = plane_lds_bytes<T>(c);
+    const size_t lds_pencil = sizeof(C) * ((size_t)c.n1 * TZ + c.n1);
+    hipError_t e;
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_g2r<T>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_r2g<T>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_xpencil_vloc<T, TZ>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pencil);
+    if (e != hipSuccess) return e;
+
+    dim3 gp(k.nplanes, nb);
+    timer_begin(c);
+    hipLaunchKernelGGL((k_plane_g2r<T>), gp, dim3(512), lds_plane, c.stream,
+                       static_cast<const C*>(psi), ld, static_cast<C*>(work), k.nplanes,
+                       static_cast<const int*>(k.plane_ptr.ptr), static_cast<const int*>(k.ent_ig.ptr),
+                       static_cast<const int*>(k.ent_pos.ptr), c.ax2, c.ax3, tw2, tw3);
+    timer_end(c, "plane_g2r");
+    int nzc = (c.n3 + TZ - 1) / TZ;
+    dim3 gx(c.n2 * nzc, nb);
+    timer_begin(c);
+    hipLaunchKernelGGL((k_xpencil_vloc<T, TZ>), gx, dim3(256), lds_pencil, c.stream,
+                       static_cast<C*>(work), k.nplanes, static_cast<const int*>(k.plane_x.ptr),
+                       static_cast<const int*>(k.x_to_plane.ptr), static_cast<const int*>(c.perm1.ptr),
+                       vl, c.ax1, c.n2, c.n3, tw1);
+    timer_end(c, "xpencil_vloc");
+    T scale = (T)(1.0 / ((double)c.n1 * c.n2 * c.n3));
+    timer_begin(c);
+    hipLaunchKernelGGL((k_plane_r2g<T>), gp, dim3(512), lds_plane, c.stream,
+                       static_cast<const C*>(work), k.nplanes,
+                       static_cast<const int*>(k.plane_ptr.ptr), static_cast<const int*>(k.ent_ig.ptr),
+                       static_cast<const int*>(k.ent_pos.ptr), static_cast<const C*>(psi), ld,
+                       static_cast<C*>(hpsi), ldo, static_cast<const double*>(k.kin.ptr), scale,
+                       accumulate ? 1 : 0, c.ax2, c.ax3, tw2, tw3);
+    timer_end(c, "plane_r2g");
+    return hipGetLastError();
+}
+
+template hipError_t launch_local<double>(Ctx&, const KptDev&, int, const void*, int64_t, void*,
+                                         int64_t, bool, void*);
+template hipError_t launch_local<float>(Ctx&, const KptDev&, int, const void*, int64_t, void*,
+                                        int64_t, bool, void*);
+
+}  // namespace effm

@@ -1,0 +1,547 @@
+// k_nonlocal.hip — nonlocal pseudopotential term and dH/dk on gfx950.
+//
+// Reference: src/H_psi/hamiltonian_callable.cpp:200-304 (V_NL psi) and
+// src/H_psi/hamiltonian_gradient.cpp:116-294 (dH/dk psi).  The reference
+// loops atom x (l, ib) x m with two passes over the G-sphere per channel and
+// band; across bands those passes are the complex GEMMs
+//   C[c][b]  = sum_g conj(P[c][g]) psi[b][g]        (<p|psi>)
+//   hpsi[b][g] += sum_c P[c][g] lambda_c C[c][b]      (sum lambda <p|psi> p)
+// which run here on the matrix cores (v_mfma_f64_16x16x4f64 for FP64,
+// v_mfma_f32_16x16x4f32 for the mixed path).  The split-K partial sums of
+// the first GEMM are reduced and scaled by lambda in FP64 (the D
+// contraction), in both precisions.
+//
+// P is generated on the device once per k-point:
+//   P[c][g] = beta_{l,ib}(q_g)/sqrt(Omega) * Y_lm(theta_g, phi_g) * S_a(g)
+// with the reference's cubic-Lagrange table rule
+// (src/pseudo/ncpp_advance.cppm:644-674), real Y_lm recurrences
+// (src/math/sph_harmonics.cppm:120-335, 702-875) and separable structure
+// factor (src/H_psi/structure_factor.cppm:92-159).
+#include "effm_internal.hpp"
+
+namespace effm {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Acc;
+template <> struct Acc<double> { using type = v4d; };
+template <> struct Acc<float>  { using type = v4f; };
+
+__device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma16(float a, float b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+//  Projector generation
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double lagrange_cubic(double f0, double f1, double f2, double f3,
+                                                 double px) {
+    double ux = 1.0 - px;
+    double vx = 2.0 - px;
+    double wx = 3.0 - px;
+    return f0 * ux * vx * wx / 6.0 + f1 * px * vx * wx / 2.0 - f2 * px * ux * wx / 2.0 +
+           f3 * px * ux * vx / 6.0;
+}
+
+__device__ __forceinline__ double interp_tab(const double* t, double dq, double q) {
+    if (q == 0.0) return t[0];
+    double s = q / dq;
+    int i0 = (int)floor(s) - 1;
+    double px = s - (double)i0;
+    if (i0 < 0) { i0 = 0; px = s; }
+    return lagrange_cubic(t[i0], t[i0 + 1], t[i0 + 2], t[i0 + 3], px);
+}
+
+__device__ __forceinline__ double seed_coeff(int m) {
+    double c = 0.5 / sqrt(M_PI);
+    if (m & 1) c = -c;
+    for (int k = 1; k <= m; ++k) c *= sqrt((double)(2 * k + 1) / (2 * k));
+    return c;
+}
+
+// Y_lm, dY/dtheta and (1/sin theta) dY/dphi for one (l, m) by the column
+// recurrences of the reference (QRecurrence, AngGradThetaRecurrence,
+// AngGradPhiRecurrence).
+__device__ void ylm_eval(int l, int m, double ct, double st, double cp, double sp, bool grad,
+                         double* Y, double* dT, double* dP) {
+    const int ma = m < 0 ? -m : m;
+    const double c0 = seed_coeff(ma);
+    double Qc = (ma == 0) ? 0.5 / sqrt(M_PI) : c0 * pow(st, (double)ma);
+    double Qp = 0.0;
+    double dQc = 0.0, dQp = 0.0, Rc = 0.0, Rp = 0.0;
+    if (grad) {
+        if (ma == 0) dQc = 0.0;
+        else if (ma == 1) dQc = seed_coeff(1) * ct;
+        else dQc = (double)ma * c0 * pow(st, (double)(ma - 1)) * ct;
+        if (ma == 1) Rc = c0;
+        else if (ma >= 2) Rc = c0 * pow(st, (double)(ma - 1));
+    }
+    if (l > ma) {
+        double f = sqrt(2.0 * ma + 3.0);
+        double Qmm = Qc;
+        Qp = Qc; Qc = f * ct * Qc;
+        if (grad) {
+            dQp = dQc; dQc = f * (-st * Qmm + ct * dQc);
+            Rp = Rc; Rc = f * ct * Rc;
+        }
+        for (int ll = ma + 2; ll <= l; ++ll) {
+            double r1 = (2.0 * ll - 1.0) *
+                        sqrt(((2.0 * ll + 1.0) * (ll - ma)) / ((2.0 * ll - 1.0) * (ll + ma)));
+            double r2 = (ll + ma - 1.0) *
+                        sqrt(((2.0 * ll + 1.0) * (ll - ma) * (ll - ma - 1)) /
+                             ((2.0 * ll - 3.0) * (ll + ma) * (ll + ma - 1)));
+            double den = 1.0 / (ll - ma);
+            double Ql1 = Qc;
+            double nq = (ct * r1 * Qc - r2 * Qp) * den;
+            Qp = Qc; Qc = nq;
+            if (grad) {
+                double nd = (ct * r1 * dQc - st * r1 * Ql1 - r2 * dQp) * den;
+                dQp = dQc; dQc = nd;
+                double nr = (ct * r1 * Rc - r2 * Rp) * den;
+                Rp = Rc; Rc = nr;
+            }
+        }
+    }
+    double cm = 1.0, sm = 0.0;
+    for (int k = 0; k < ma; ++k) {
+        double oc = cm;
+        cm = cm * cp - sm * sp;
+        sm = sm * cp + oc * sp;
+    }
+    const double r2 = sqrt(2.0);
+    if (m == 0) *Y = Qc;
+    else *Y = (m > 0) ? (r2 * Qc) * cm : (r2 * Qc) * sm;
+    if (grad) {
+        if (m == 0) *dT = dQc;
+        else *dT = (m > 0) ? r2 * dQc * cm : r2 * dQc * sm;
+        if (m == 0) *dP = 0.0;
+        else if (m > 0) *dP = (-(double)m * r2) * Rc * sm;
+        else *dP = ((double)ma * r2) * Rc * cm;
+    }
+}
+
+__device__ __forceinline__ double2 cmul_d(double2 a, double2 b) {
+    return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+template <bool GRAD>
+__global__ void __launch_bounds__(256)
+k_build_proj(const double* __restrict__ K, const int* __restrict__ gidx, int npw, double kf0,
+             double kf1, double kf2, const ChanDev* __restrict__ chans,
+             const TableDev* __restrict__ tinfo, const double* __restrict__ btab,
+             const double* __restrict__ dbtab, int nq_stride, const double* __restrict__ tau,
+             double2* __restrict__ P64, float2* __restrict__ P32, double2* __restrict__ dP64,
+             float2* __restrict__ dP32, int nch) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (g >= npw) return;
+    const ChanDev ch = chans[c];
+    const double kx = K[g], ky = K[npw + g], kz = K[2 * npw + g];
+    // GKK::computeSpherical (src/io/GKK.cppm:431-449)
+    const double q = sqrt(kx * kx + ky * ky + kz * kz);
+    const double th = (q > 0.0) ? acos(kz / q) : 0.0;
+    const double ph = atan2(ky, kx);
+    const double ct = cos(th), st = sin(th), cp = cos(ph), sp = sin(ph);
+    double Y, dT = 0.0, dPh = 0.0;
+    ylm_eval(ch.l, ch.m, ct, st, cp, sp, GRAD, &Y, &dT, &dPh);
+    const TableDev t = tinfo[ch.table];
+    const double beta = interp_tab(btab + (size_t)ch.table * nq_stride, t.dq, q) * t.norm;
+    // S = exp(-2 pi i k.tau) p_x[g_x] p_y[g_y] p_z[g_z]
+    const double* ta = tau + 3 * ch.atom;
+    double s, co;
+    sincospi(-2.0 * (kf0 * ta[0] + kf1 * ta[1] + kf2 * ta[2]), &s, &co);
+    double2 S = double2{co, s};
+    for (int d = 0; d < 3; ++d) {
+        sincospi(-2.0 * (double)gidx[3 * g + d] * ta[d], &s, &co);
+        S = cmul_d(S, double2{co, s});
+    }
+    const double by = beta * Y;
+    const size_t o = (size_t)c * npw + g;
+    const double2 p = double2{by * S.x, by * S.y};
+    if (P64) P64[o] = p;
+    if (P32) P32[o] = float2{(float)p.x, (float)p.y};
+    if (GRAD) {
+        const double dbeta = interp_tab(dbtab + (size_t)ch.table * nq_stride, t.dq, q) * t.norm;
+        const double bo = (q > 1e-12) ? beta / q : t.dbeta0;
+        const double gq = dbeta * Y;
+        const double gt = bo * dT;
+        const double gp = bo * dPh;
+        const double r[3] = {st * cp * gq + ct * cp * gt - sp * gp,
+                             st * sp * gq + ct * sp * gt + cp * gp,
+                             ct * gq - st * gt};
+        for (int a = 0; a < 3; ++a) {
+            const size_t oa = ((size_t)a * nch + c) * npw + g;
+            const double2 v = double2{r[a] * S.x, r[a] * S.y};
+            if (dP64) dP64[oa] = v;
+            if (dP32) dP32[oa] = float2{(float)v.x, (float)v.y};
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_build_proj(Ctx& c, KptDev& k, bool grad) {
+    const int nch = c.nch();
+    if (nch == 0) return hipSuccess;
+    const bool f64 = sizeof(T) == 8;
+    dim3 grid((k.npw + 255) / 256, nch);
+    timer_begin(c);
+    if (grad)
+        hipLaunchKernelGGL((k_build_proj<true>), grid, dim3(256), 0, c.stream,
+                           static_cast<const double*>(k.K.ptr), static_cast<const int*>(k.gidx.ptr),
+                           k.npw, k.kfrac[0], k.kfrac[1], k.kfrac[2],
+                           static_cast<const ChanDev*>(c.chan_dev.ptr),
+                           static_cast<const TableDev*>(c.tab_info.ptr),
+                           static_cast<const double*>(c.btab.ptr), static_cast<const double*>(c.dbtab.ptr),
+                           c.nq_max, static_cast<const double*>(c.tau.ptr),
+                           f64 ? static_cast<double2*>(k.P64.ptr) : nullptr,
+                           f64 ? nullptr : static_cast<float2*>(k.P32.ptr),
+                           f64 ? static_cast<double2*>(k.dP64.ptr) : nullptr,
+                           f64 ? nullptr : static_cast<float2*>(k.dP32.ptr), nch);
+    else
+        hipLaunchKernelGGL((k_build_proj<false>), grid, dim3(256), 0, c.stream,
+                           static_cast<const double*>(k.K.ptr), static_cast<const int*>(k.gidx.ptr),
+                           k.npw, k.kfrac[0], k.kfrac[1], k.kfrac[2],
+                           static_cast<const ChanDev*>(c.chan_dev.ptr),
+                           static_cast<const TableDev*>(c.tab_info.ptr),
+                           static_cast<const double*>(c.btab.ptr), nullptr, c.nq_max,
+                           static_cast<const double*>(c.tau.ptr),
+                           f64 ? static_cast<double2*>(k.P64.ptr) : nullptr,
+                           f64 ? nullptr : static_cast<float2*>(k.P32.ptr), nullptr, nullptr, nch);
+    timer_end(c, "proj_build");
+    return hipGetLastError();
+}
+
+template hipError_t launch_build_proj<double>(Ctx&, KptDev&, bool);
+template hipError_t launch_build_proj<float>(Ctx&, KptDev&, bool);
+
+// ---------------------------------------------------------------------------
+//  GEMM 1: part[split][row0 + m][n] = sum_{k in split} conj(A[m][k]) B[n][k]
+//  A: [M][lda] (projectors, G contiguous), B: [N][ldb] (bands, G contiguous).
+//  Workgroup = 4 waves, 64 x 64 tile; wave = 2 x 2 blocks of 16 x 16.
+//  Each lane owns 4 consecutive k of its row per 16-wide k step, so both
+//  operands are read as contiguous 4-element runs.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemm_proj(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restrict__ B,
+            int64_t ldb, int M, int N, int K, int kchunk, cx_t<T>* __restrict__ part,
+            int Mtot, int row0) {
+    using C = cx_t<T>;
+    using V = typename Acc<T>::type;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = lane & 15, kk = lane >> 4;
+    const int m0 = blockIdx.x * 64 + (wave & 1) * 32;
+    const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
+    const int ks = blockIdx.z * kchunk;
+    const int ke = min(K, ks + kchunk);
+    V accR[2][2], accI[2][2];
+    for (int r = 0; r < 2; ++r)
+        for (int q = 0; q < 2; ++q) { accR[r][q] = V{0, 0, 0, 0}; accI[r][q] = V{0, 0, 0, 0}; }
+    const C* ar[2];
+    const C* br[2];
+    bool av[2], bv[2];
+    for (int r = 0; r < 2; ++r) {
+        int row = m0 + r * 16 + i;
+        av[r] = row < M;
+        ar[r] = A + (int64_t)(av[r] ? row : 0) * lda;
+        int col = n0 + r * 16 + i;
+        bv[r] = col < N;
+        br[r] = B + (int64_t)(bv[r] ? col : 0) * ldb;
+    }
+    for (int kb = ks; kb < ke; kb += 16) {
+        C a[2][4], b[2][4];
+        const int kl = kb + kk * 4;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const bool kv = (kl + s) < ke;
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                a[r][s] = (av[r] && kv) ? ar[r][kl + s] : C{0, 0};
+                b[r][s] = (bv[r] && kv) ? br[r][kl + s] : C{0, 0};
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    accR[r][q] = mfma16(a[r][s].x, b[q][s].x, accR[r][q]);
+                    accR[r][q] = mfma16(a[r][s].y, b[q][s].y, accR[r][q]);
+                    accI[r][q] = mfma16(a[r][s].x, b[q][s].y, accI[r][q]);
+                    accI[r][q] = mfma16(-a[r][s].y, b[q][s].x, accI[r][q]);
+                }
+    }
+    C* po = part + (int64_t)blockIdx.z * Mtot * N;
+    for (int r = 0; r < 2; ++r)
+        for (int q = 0; q < 2; ++q)
+            for (int v = 0; v < 4; ++v) {
+                int row = m0 + r * 16 + kk * 4 + v;
+                int col = n0 + q * 16 + i;
+                if (row < M && col < N)
+                    po[(int64_t)(row0 + row) * N + col] = C{accR[r][q][v], accI[r][q][v]};
+            }
+}
+
+// out[m][n] = lambda[m % nch] * sum_s part[s][m][n], summed in FP64.
+template <typename T>
+__global__ void k_reduce_scale(const cx_t<T>* __restrict__ part, int nsplit, int Mtot, int N,
+                               const double* __restrict__ lam, int nch, cx_t<T>* __restrict__ out) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t tot = (int64_t)Mtot * N;
+    if (idx >= tot) return;
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < nsplit; ++s) {
+        cx_t<T> v = part[(int64_t)s * tot + idx];
+        re += (double)v.x;
+        im += (double)v.y;
+    }
+    const int m = (int)(idx / N);
+    const double l = lam[m % nch];
+    out[idx] = cx_t<T>{(T)(re * l), (T)(im * l)};
+}
+
+// ---------------------------------------------------------------------------
+//  GEMM 2: out[n][m] (+)= diag[m] * src[n][m] + sum_k A[k][m] B[k][n]
+//  A: [K][lda] (projectors, G contiguous), B: [K][ldb] (scaled <p|psi>).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemm_apply(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restrict__ B,
+             int64_t ldb, int M, int N, int K, cx_t<T>* __restrict__ out, int64_t ldo,
+             int accumulate, const double* __restrict__ diag, const cx_t<T>* __restrict__ src,
+             int64_t lds) {
+    using C = cx_t<T>;
+    using V = typename Acc<T>::type;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = lane & 15, kk = lane >> 4;
+    const int m0 = blockIdx.x * 64 + (wave & 1) * 32;
+    const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
+    V accR[2][2], accI[2][2];
+    for (int r = 0; r < 2; ++r)
+        for (int q = 0; q < 2; ++q) { accR[r][q] = V{0, 0, 0, 0}; accI[r][q] = V{0, 0, 0, 0}; }
+    bool mv[2], nv[2];
+    for (int r = 0; r < 2; ++r) {
+        mv[r] = (m0 + r * 16 + i) < M;
+        nv[r] = (n0 + r * 16 + i) < N;
+    }
+    for (int k0 = 0; k0 < K; k0 += 4) {
+        const int k = k0 + kk;
+        const bool kv = k < K;
+        C a[2], b[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            a[r] = (kv && mv[r]) ? A[(int64_t)k * lda + m0 + r * 16 + i] : C{0, 0};
+            b[r] = (kv && nv[r]) ? B[(int64_t)k * ldb + n0 + r * 16 + i] : C{0, 0};
+        }
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                accR[r][q] = mfma16(a[r].x, b[q].x, accR[r][q]);
+                accR[r][q] = mfma16(-a[r].y, b[q].y, accR[r][q]);
+                accI[r][q] = mfma16(a[r].x, b[q].y, accI[r][q]);
+                accI[r][q] = mfma16(a[r].y, b[q].x, accI[r][q]);
+            }
+    }
+    for (int r = 0; r < 2; ++r)
+        for (int q = 0; q < 2; ++q) {
+            const int n = n0 + q * 16 + i;
+            if (n >= N) continue;
+            for (int v = 0; v < 4; ++v) {
+                const int m = m0 + r * 16 + kk * 4 + v;
+                if (m >= M) continue;
+                double re = (double)accR[r][q][v], im = (double)accI[r][q][v];
+                if (diag) {
+                    C s = src[(int64_t)n * lds + m];
+                    re += diag[m] * (double)s.x;
+                    im += diag[m] * (double)s.y;
+                }
+                C* o = out + (int64_t)n * ldo + m;
+                if (accumulate) {
+                    C h = *o;
+                    re += (double)h.x;
+                    im += (double)h.y;
+                }
+                *o = C{(T)re, (T)im};
+            }
+        }
+}
+
+static int choose_split(int M, int N, int K) {
+    int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+    int want = (1024 + tiles - 1) / tiles;
+    int maxs = (K + 255) / 256;
+    if (want > maxs) want = maxs;
+    if (want < 1) want = 1;
+    return want;
+}
+
+size_t vnl_partial_bytes(int nrows, int nb, int npw, size_t elem) {
+    int ns = choose_split(nrows, nb, npw);
+    return (size_t)ns * nrows * nb * elem;
+}
+
+template <typename T>
+static hipError_t gemm_proj(Ctx& c, const cx_t<T>* A, int M, int npw, const cx_t<T>* psi,
+                            int64_t ld, int nb, cx_t<T>* part, int Mtot, int row0, int nsplit) {
+    int kchunk = (npw + nsplit - 1) / nsplit;
+    kchunk = (kchunk + 15) / 16 * 16;
+    dim3 grid((M + 63) / 64, (nb + 63) / 64, nsplit);
+    hipLaunchKernelGGL((k_gemm_proj<T>), grid, dim3(256), 0, c.stream, A, (int64_t)npw, psi, ld,
+                       M, nb, npw, kchunk, part, Mtot, row0);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t gemm_apply(Ctx& c, const cx_t<T>* A, int npw, const cx_t<T>* B, int nb,
+                             int K, cx_t<T>* out, int64_t ldo, bool acc, const double* diag,
+                             const cx_t<T>* src, int64_t lds) {
+    dim3 grid((npw + 63) / 64, (nb + 63) / 64);
+    hipLaunchKernelGGL((k_gemm_apply<T>), grid, dim3(256), 0, c.stream, A, (int64_t)npw, B,
+                       (int64_t)nb, npw, nb, K, out, ldo, acc ? 1 : 0, diag, src, lds);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_vnl(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld, void* hpsi,
+                      int64_t ldo, void* partial, size_t partial_bytes, void* cbuf) {
+    using C = cx_t<T>;
+    const int nch = c.nch();
+    if (nch == 0) return hipSuccess;
+    const bool f64 = sizeof(T) == 8;
+    const C* P = static_cast<const C*>(f64 ? k.P64.ptr : k.P32.ptr);
+    const int nsplit = choose_split(nch, nb, k.npw);
+    if ((size_t)nsplit * nch * nb * sizeof(C) > partial_bytes) return hipErrorInvalidValue;
+    timer_begin(c);
+    hipError_t e = gemm_proj<T>(c, P, nch, k.npw, static_cast<const C*>(psi), ld, nb,
+                                static_cast<C*>(partial), nch, 0, nsplit);
+    timer_end(c, "proj_gemm");
+    if (e != hipSuccess) return e;
+    int64_t tot = (int64_t)nch * nb;
+    timer_begin(c);
+    hipLaunchKernelGGL((k_reduce_scale<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       c.stream, static_cast<const C*>(partial), nsplit, nch, nb,
+                       static_cast<const double*>(c.lam64.ptr), nch, static_cast<C*>(cbuf));
+    timer_end(c, "proj_reduce");
+    timer_begin(c);
+    e = gemm_apply<T>(c, P, k.npw, static_cast<const C*>(cbuf), nb, nch, static_cast<C*>(hpsi),
+                      ldo, true, nullptr, nullptr, 0);
+    timer_end(c, "proj_apply");
+    return e;
+}
+
+template <typename T>
+hipError_t launch_dhdk(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld, void* ox,
+                       void* oy, void* oz, int64_t ldo, bool accumulate, void* partial,
+                       size_t partial_bytes, void* cbuf) {
+    using C = cx_t<T>;
+    const int nch = c.nch();
+    const bool f64 = sizeof(T) == 8;
+    void* outs[3] = {ox, oy, oz};
+    const double* Kd = static_cast<const double*>(k.K.ptr);
+    hipError_t e = hipSuccess;
+    if (nch == 0) {
+        // kinetic part only: out_a (+)= K_a psi  (gradient.cpp:139-141)
+        for (int a = 0; a < 3; ++a) {
+            if (!outs[a]) continue;
+            e = gemm_apply<T>(c, nullptr, k.npw, nullptr, nb, 0, static_cast<C*>(outs[a]), ldo,
+                              accumulate, Kd + (size_t)a * k.npw, static_cast<const C*>(psi), ld);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    const C* P = static_cast<const C*>(f64 ? k.P64.ptr : k.P32.ptr);
+    const C* dP = static_cast<const C*>(f64 ? k.dP64.ptr : k.dP32.ptr);
+    const int Mtot = 4 * nch;
+    const int nsplit = choose_split(Mtot, nb, k.npw);
+    if ((size_t)nsplit * Mtot * nb * sizeof(C) > partial_bytes) return hipErrorInvalidValue;
+    // <p|psi> and <dp_a|psi> for a = x, y, z in one partial buffer
+    timer_begin(c);
+    e = gemm_proj<T>(c, P, nch, k.npw, static_cast<const C*>(psi), ld, nb,
+                     static_cast<C*>(partial), Mtot, 0, nsplit);
+    if (e == hipSuccess)
+        e = gemm_proj<T>(c, dP, 3 * nch, k.npw, static_cast<const C*>(psi), ld, nb,
+                         static_cast<C*>(partial), Mtot, nch, nsplit);
+    timer_end(c, "proj_gemm");
+    if (e != hipSuccess) return e;
+    int64_t tot = (int64_t)Mtot * nb;
+    timer_begin(c);
+    hipLaunchKernelGGL((k_reduce_scale<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       c.stream, static_cast<const C*>(partial), nsplit, Mtot, nb,
+                       static_cast<const double*>(c.lam64.ptr), nch, static_cast<C*>(cbuf));
+    timer_end(c, "proj_reduce");
+    const C* cb = static_cast<const C*>(cbuf);
+    timer_begin(c);
+    for (int a = 0; a < 3 && e == hipSuccess; ++a) {
+        if (!outs[a]) continue;
+        C* o = static_cast<C*>(outs[a]);
+        // out_a (+)= K_a psi + sum_c p_c lambda_c <dp_a,c|psi>
+        e = gemm_apply<T>(c, P, k.npw, cb + (size_t)(1 + a) * nch * nb, nb, nch, o, ldo,
+                          accumulate, Kd + (size_t)a * k.npw, static_cast<const C*>(psi), ld);
+        // out_a += sum_c dp_a,c lambda_c <p_c|psi>
+        if (e == hipSuccess)
+            e = gemm_apply<T>(c, dP + (size_t)a * nch * k.npw, k.npw, cb, nb, nch, o, ldo, true,
+                              nullptr, nullptr, 0);
+    }
+    timer_end(c, "proj_apply");
+    return e;
+}
+
+template hipError_t launch_vnl<double>(Ctx&, const KptDev&, int, const void*, int64_t, void*,
+                                       int64_t, void*, size_t, void*);
+template hipError_t launch_vnl<float>(Ctx&, const KptDev&, int, const void*, int64_t, void*,
+                                      int64_t, void*, size_t, void*);
+template hipError_t launch_dhdk<double>(Ctx&, const KptDev&, int, const void*, int64_t, void*,
+                                        void*, void*, int64_t, bool, void*, size_t, void*);
+template hipError_t launch_dhdk<float>(Ctx&, const KptDev&, int, const void*, int64_t, void*,
+                                       void*, void*, int64_t, bool, void*, size_t, void*);
+
+// ---------------------------------------------------------------------------
+//  FP64 Rayleigh quotients
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_rayleigh(const cx_t<T>* __restrict__ psi, int64_t ld, const cx_t<T>* __restrict__ h,
+           int64_t ldh, int npw, double* __restrict__ E) {
+    __shared__ double sn[256], sd[256];
+    const int b = blockIdx.x;
+    double num = 0.0, den = 0.0;
+    for (int g = threadIdx.x; g < npw; g += blockDim.x) {
+        cx_t<T> p = psi[(int64_t)b * ld + g];
+        cx_t<T> q = h[(int64_t)b * ldh + g];
+        num += (double)p.x * (double)q.x + (double)p.y * (double)q.y;
+        den += (double)p.x * (double)p.x + (double)p.y * (double)p.y;
+    }
+    sn[threadIdx.x] = num;
+    sd[threadIdx.x] = den;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sn[threadIdx.x] += sn[threadIdx.x + s];
+            sd[threadIdx.x] += sd[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) E[b] = sn[0] / sd[0];
+}
+
+template <typename T>
+hipError_t launch_rayleigh(hipStream_t s, int npw, int nb, const void* psi, int64_t ld,
+                           const void* hpsi, int64_t ldh, double* E) {
+    hipLaunchKernelGGL((k_rayleigh<T>), dim3(nb), dim3(256), 0, s,
+                       static_cast<const cx_t<T>*>(psi), ld, static_cast<const cx_t<T>*>(hpsi), ldh,
+                       npw, E);
+    return hipGetLastError();
+}
+
+template hipError_t launch_rayleigh<double>(hipStream_t, int, int, const void*, int64_t,
+                                            const void*, int64_t, double*);
+template hipError_t launch_rayleigh<float>(hipStream_t, int, int, const void*, int64_t,
+                                           const void*, int64_t, double*);
+
+}  // namespace effm

@@ -1,0 +1,301 @@
+"""Python mirror of the reference's H_psi module interface over libeffm_hip.so.
+
+Reference: src/H_psi/hamiltonian.cppm:115-363.
+  Hamiltonian.at_k(ikpt) -> Callable;  Callable(psi, hpsi)  (hpsi += H psi)
+  Callable(psi, hpsi, n_bands=nb)      the declared batched form (:180-182)
+  Hamiltonian.gradient().at_k(ikpt)(psi, GradHPsi(x, y, z))  (:218-263)
+Semantics kept from the reference: results ACCUMULATE into the output
+(src/H_psi/hamiltonian_callable.cpp:87-92); a None direction of GradHPsi is
+skipped (hamiltonian.cppm:53-65).  Errors raise the classes of _lib that
+subclass the reference's exception kinds (runtime / out_of_range /
+domain_error).
+
+psi / hpsi may be torch tensors on the GPU (complex128 for "fp64",
+complex64 for "mixed"; shape [nb, npw] or [npw], last dim contiguous) or
+numpy arrays on the host (staged through device buffers).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Any, Optional
+
+import numpy as np
+
+from . import _lib
+from .synth import Cell as SynthCell, KPoint as SynthKPoint
+
+PREC = {"fp64": _lib.EFFM_PREC_FP64, "mixed": _lib.EFFM_PREC_MIXED}
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _as_2d_info(x, prec: int):
+    """(pointer, nb, ld, host?) of a band-major psi-like array."""
+    want = np.complex128 if prec == _lib.EFFM_PREC_FP64 else np.complex64
+    if _is_torch(x):
+        import torch
+        tdt = torch.complex128 if prec == _lib.EFFM_PREC_FP64 else torch.complex64
+        if x.dtype != tdt:
+            raise TypeError(f"expected {tdt}, got {x.dtype}")
+        if not x.is_cuda:
+            raise TypeError("torch tensors must live on the GPU; pass numpy arrays for host data")
+        if x.dim() == 1:
+            x2 = x.view(1, -1)
+        else:
+            x2 = x
+        if x2.stride(-1) != 1:
+            raise ValueError("last dimension must be contiguous")
+        return x.data_ptr(), x2.shape[0], x2.stride(0) if x2.shape[0] > 1 else x2.shape[1], False
+    if isinstance(x, np.ndarray):
+        if x.dtype != want:
+            raise TypeError(f"expected {np.dtype(want)}, got {x.dtype}")
+        if not x.flags.c_contiguous:
+            raise ValueError("host arrays must be C-contiguous")
+        x2 = x.reshape(1, -1) if x.ndim == 1 else x
+        return x.ctypes.data, x2.shape[0], x2.shape[1], True
+    raise TypeError(f"unsupported array type {type(x)}")
+
+
+@dataclass
+class GradHPsi:
+    """Output container of dH/dk|psi> (hamiltonian.cppm:66-70); None = skip."""
+    x: Any = None
+    y: Any = None
+    z: Any = None
+
+
+class Hamiltonian:
+    """Owns the device-resident cell and k-points (hamiltonian.cppm:115)."""
+
+    def __init__(self, cell: SynthCell, device: int = 0, hbm_budget: int = 0):
+        self._lib = _lib.load()
+        self._ctx = C.c_void_p()
+        _lib.check(self._lib.effm_ctx_create(C.byref(self._ctx), device, hbm_budget), None,
+                   "effm_ctx_create")
+        self.device = device
+        self._keep = []
+        self.cell = cell
+        self._upload_cell(cell)
+        self._npw = {}
+
+    # -- lifetime --------------------------------------------------------
+    def close(self):
+        if self._ctx:
+            self._lib.effm_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    def _sync_stream(self):
+        import torch
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        self._lib.effm_set_stream(self._ctx, C.c_void_p(s))
+
+    # -- data --------------------------------------------------------------
+    def _upload_cell(self, cell: SynthCell):
+        keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a
+
+        sps = (_lib.Species * max(1, len(cell.species_list)))()
+        for i, sp in enumerate(cell.species_list):
+            bl = arr(sp.beta_l, np.int32)
+            lam = arr(sp.lam, np.float64)
+            bt = arr(sp.beta_tab, np.float64)
+            dbt = arr(sp.dbeta_tab, np.float64) if sp.dbeta_tab is not None else None
+            sps[i].lmax = sp.lmax
+            sps[i].nbeta = sp.nbeta
+            sps[i].beta_l = bl.ctypes.data_as(C.POINTER(C.c_int32))
+            sps[i].lam = lam.ctypes.data_as(C.POINTER(C.c_double))
+            sps[i].nq = sp.nq
+            sps[i].dq = sp.dq
+            sps[i].qmax = sp.qmax
+            sps[i].beta_tab = bt.ctypes.data_as(C.POINTER(C.c_double))
+            sps[i].dbeta_tab = dbt.ctypes.data_as(C.POINTER(C.c_double)) if dbt is not None else None
+        vl = arr(cell.vloc, np.float64)
+        spc = arr(cell.species, np.int32)
+        tau = arr(cell.tau, np.float64)
+        cc = _lib.Cell()
+        cc.n1, cc.n2, cc.n3 = cell.n
+        for i, v in enumerate(np.asarray(cell.A, dtype=np.float64).ravel()):
+            cc.lattice[i] = v
+        cc.vloc = vl.ctypes.data_as(C.POINTER(C.c_double))
+        cc.natom = cell.natom
+        cc.species = spc.ctypes.data_as(C.POINTER(C.c_int32))
+        cc.tau = tau.ctypes.data_as(C.POINTER(C.c_double))
+        cc.nspecies = len(cell.species_list)
+        cc.sp = sps
+        _lib.check(self._lib.effm_cell_upload(self._ctx, C.byref(cc)), self._ctx, "effm_cell_upload")
+
+    def load_kpoint(self, ikpt: int, kp: SynthKPoint):
+        """GKK::loadKPoint equivalent: make k-point ikpt resident."""
+        K = np.ascontiguousarray(kp.K, dtype=np.float64)
+        kin = np.ascontiguousarray(kp.kin, dtype=np.float64)
+        gi = np.ascontiguousarray(kp.gidx, dtype=np.int32)
+        kf = np.ascontiguousarray(kp.kfrac, dtype=np.float64)
+        _lib.check(self._lib.effm_kpt_upload(self._ctx, ikpt, kp.npw, K.ctypes.data, kin.ctypes.data,
+                                             gi.ctypes.data, kf.ctypes.data), self._ctx,
+                   "effm_kpt_upload")
+        self._npw[ikpt] = kp.npw
+
+    def release_kpoint(self, ikpt: int):
+        _lib.check(self._lib.effm_kpt_release(self._ctx, ikpt), self._ctx, "effm_kpt_release")
+        self._npw.pop(ikpt, None)
+
+    def mem_info(self):
+        u, b = C.c_uint64(), C.c_uint64()
+        self._lib.effm_mem_info(self._ctx, C.byref(u), C.byref(b))
+        return u.value, b.value
+
+    # -- operators -----------------------------------------------------------
+    def at_k(self, ikpt: int) -> "Callable":
+        return Callable(self, ikpt)
+
+    def gradient(self) -> "Gradient":
+        return Gradient(self)
+
+    def rayleigh(self, psi, hpsi, prec: str = "fp64"):
+        """FP64 E_b = Re<psi_b|hpsi_b>/<psi_b|psi_b> (test_hpsi_eigen.cpp:49-56)."""
+        p = PREC[prec]
+        pp, nb, ld, host = _as_2d_info(psi, p)
+        hp, nb2, ldh, host2 = _as_2d_info(hpsi, p)
+        npw = (psi.shape[-1])
+        if host:
+            E = np.zeros(nb, dtype=np.float64)
+            _lib.check(self._lib.effm_rayleigh(self._ctx, npw, nb, pp, ld, hp, ldh, p, E.ctypes.data,
+                                               _lib.EFFM_F_HOST), self._ctx, "effm_rayleigh")
+            return E
+        import torch
+        self._sync_stream()
+        E = torch.zeros(nb, dtype=torch.float64, device=psi.device)
+        _lib.check(self._lib.effm_rayleigh(self._ctx, npw, nb, pp, ld, hp, ldh, p, E.data_ptr(), 0),
+                   self._ctx, "effm_rayleigh")
+        return E
+
+    def apply_multi(self, ikpts, psis, hpsis, prec: str = "fp64", overwrite: bool = False):
+        """H|psi> for several k-points (ragged npw) in one call; device tensors."""
+        p = PREC[prec]
+        nk = len(ikpts)
+        ks = (C.c_int * nk)(*ikpts)
+        pp = (C.c_void_p * nk)()
+        hp = (C.c_void_p * nk)()
+        lds = (C.c_int64 * nk)()
+        nb = None
+        for i in range(nk):
+            a, nbi, ld, host = _as_2d_info(psis[i], p)
+            b, nbo, ldo, host_o = _as_2d_info(hpsis[i], p)
+            if host or host_o or ld != ldo:
+                raise ValueError("apply_multi takes device tensors with equal leading dimensions")
+            if nb is not None and nbi != nb:
+                raise ValueError("every k-point needs the same band count")
+            nb = nbi
+            pp[i], hp[i], lds[i] = a, b, ld
+        self._sync_stream()
+        flags = _lib.EFFM_F_OVERWRITE if overwrite else 0
+        _lib.check(self._lib.effm_hpsi_multi(self._ctx, nk, ks, nb or 0, pp, lds, hp, p, flags),
+                   self._ctx, "effm_hpsi_multi")
+        return hpsis
+
+    def set_timing(self, on: bool):
+        self._lib.effm_set_timing(self._ctx, 1 if on else 0)
+
+    def timing(self, name: str):
+        ms, cnt = C.c_double(), C.c_int64()
+        self._lib.effm_get_timing(self._ctx, name.encode(), C.byref(ms), C.byref(cnt))
+        return ms.value, cnt.value
+
+    def reset_timing(self):
+        self._lib.effm_reset_timing(self._ctx)
+
+
+class Callable:
+    """H|psi> at fixed k (hamiltonian.cppm:173-204)."""
+
+    def __init__(self, parent: Hamiltonian, ikpt: int):
+        self._h = parent
+        self._ikpt = ikpt
+        self.dim()   # raises out_of_range for an unknown k-point
+
+    def dim(self) -> int:
+        return _lib.check(self._h._lib.effm_kpt_dim(self._h._ctx, self._ikpt), self._h._ctx,
+                          "Callable::dim")
+
+    def set_ikpt(self, ikpt: int):
+        self._ikpt = ikpt
+        self.dim()
+
+    def __call__(self, psi, hpsi, n_bands: Optional[int] = None, prec: str = "fp64",
+                 overwrite: bool = False):
+        p = PREC[prec]
+        pp, nb, ld, host = _as_2d_info(psi, p)
+        hp, nbo, ldo, host_o = _as_2d_info(hpsi, p)
+        if host != host_o:
+            raise TypeError("psi and hpsi must both be host or both be device arrays")
+        if n_bands is not None:
+            nb = n_bands
+        flags = (_lib.EFFM_F_OVERWRITE if overwrite else 0) | (_lib.EFFM_F_HOST if host else 0)
+        if not host:
+            self._h._sync_stream()
+        _lib.check(self._h._lib.effm_hpsi(self._h._ctx, self._ikpt, nb, pp, ld, hp, ldo, p, flags),
+                   self._h._ctx, "H|psi>")
+        return hpsi
+
+
+class Gradient:
+    def __init__(self, parent: Hamiltonian):
+        self._h = parent
+
+    def at_k(self, ikpt: int) -> "GradientCallable":
+        return GradientCallable(self._h, ikpt)
+
+
+class GradientCallable:
+    """dH/dk_alpha |psi> at fixed k (hamiltonian.cppm:222-254)."""
+
+    def __init__(self, parent: Hamiltonian, ikpt: int):
+        self._h = parent
+        self._ikpt = ikpt
+        self.dim()
+
+    def dim(self) -> int:
+        return _lib.check(self._h._lib.effm_kpt_dim(self._h._ctx, self._ikpt), self._h._ctx,
+                          "Gradient::Callable::dim")
+
+    def __call__(self, psi, out: GradHPsi, prec: str = "fp64", overwrite: bool = False):
+        p = PREC[prec]
+        pp, nb, ld, host = _as_2d_info(psi, p)
+        ptrs = []
+        ldo = None
+        for o in (out.x, out.y, out.z):
+            if o is None:
+                ptrs.append(None)
+                continue
+            op, nbo, l2, host_o = _as_2d_info(o, p)
+            if host_o != host:
+                raise TypeError("psi and outputs must both be host or both be device arrays")
+            if ldo is not None and l2 != ldo:
+                raise ValueError("all outputs need the same leading dimension")
+            ldo = l2
+            ptrs.append(op)
+        if ldo is None:
+            return out
+        flags = (_lib.EFFM_F_OVERWRITE if overwrite else 0) | (_lib.EFFM_F_HOST if host else 0)
+        if not host:
+            self._h._sync_stream()
+        _lib.check(self._h._lib.effm_dhdk(self._h._ctx, self._ikpt, nb, pp, ld, ptrs[0], ptrs[1], ptrs[2],
+                                          ldo, p, flags), self._h._ctx, "dH/dk|psi>")
+        return out

@@ -1,0 +1,154 @@
+/*
+ * effm_hpsi.h — C-ABI of libeffm_hip.so, the MI355X (gfx950) H|psi> engine.
+ *
+ * Plain C: pointers, sizes and status codes, no C++ or torch types.  The
+ * reference has no FFI (it is a C++23 module library); each entry point below
+ * replaces the reference interface cited beside it, and the C++ / Python
+ * mirrors of the reference classes (effmass_amd/host/effm_hamiltonian.hpp,
+ * effmass_amd/hamiltonian.py) are thin layers over these calls.
+ *
+ * Conventions (identical to the reference):
+ *   - Hartree atomic units; K = k + G in Cartesian bohr^-1;
+ *   - psi / hpsi are band-major: band b starts at psi + b*ld (complex
+ *     elements), G in the caller's (GKK) order;
+ *   - complex numbers are interleaved (re, im): complex128 for EFFM_PREC_FP64,
+ *     complex64 for EFFM_PREC_MIXED;
+ *   - without EFFM_F_OVERWRITE results are ACCUMULATED (hpsi += H psi), as
+ *     Hamiltonian::Callable::operator() does
+ *     (src/H_psi/hamiltonian_callable.cpp:87-92);
+ *   - array arguments of effm_hpsi / effm_dhdk / effm_rayleigh are device
+ *     pointers unless EFFM_F_HOST is set;
+ *   - all work is ordered on the context's stream (effm_set_stream).
+ */
+#ifndef EFFM_HPSI_H
+#define EFFM_HPSI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: map onto the reference's exception classes ---------- */
+#define EFFM_OK                 0
+#define EFFM_ERR_RUNTIME       -1   /* std::runtime_error  */
+#define EFFM_ERR_OUT_OF_RANGE  -2   /* std::out_of_range   */
+#define EFFM_ERR_DOMAIN        -3   /* std::domain_error   */
+#define EFFM_ERR_INVALID       -4   /* std::invalid_argument */
+#define EFFM_ERR_NOMEM         -5   /* HBM budget or hipMalloc failure */
+#define EFFM_ERR_HIP           -6   /* any other HIP runtime error */
+
+/* ---- precision ---------------------------------------------------------- */
+#define EFFM_PREC_FP64   0   /* complex128 psi; FP64 FFT, GEMM, reductions */
+#define EFFM_PREC_MIXED  1   /* complex64 psi; FP32 FFT + GEMM with FP32
+                                accumulation; FP64 D contraction, Rayleigh */
+
+/* ---- flags -------------------------------------------------------------- */
+#define EFFM_F_OVERWRITE  0x1  /* out = H psi instead of out += H psi */
+#define EFFM_F_HOST       0x2  /* psi / outputs are host pointers */
+
+typedef struct effm_ctx effm_ctx;
+
+/* One pseudopotential species after sortByL + diagonalizeNonlocal.
+ * Replaces the NCPP + BetaqTables / DBetaqTables pair the Hamiltonian holds
+ * per element (src/H_psi/hamiltonian.cppm:325;
+ * src/pseudo/ncpp_advance.cppm:522-701, 718-895). */
+typedef struct {
+    int32_t lmax;              /* -1: no projectors */
+    int32_t nbeta;             /* radial channels, sorted by l */
+    const int32_t* beta_l;     /* [nbeta] */
+    const double* lambda;      /* [nbeta] eigenvalues of the l-blocks of D (Ha) */
+    int32_t nq;                /* table nodes q_i = i*dq */
+    double dq;
+    double qmax;
+    const double* beta_tab;    /* [nbeta*nq] raw beta(q_i), no 1/sqrt(Omega) */
+    const double* dbeta_tab;   /* [nbeta*nq] raw dbeta/dq(q_i), or NULL */
+} effm_species;
+
+/* The k-independent cell data: replaces Hamiltonian::finalize()'s products
+ * (src/H_psi/hamiltonian.cpp:267-392): FFT grid, V_loc (VR, src/io/RHO.cppm:
+ * 57-61), atoms in ascending-Z / stable order (src/io/ATOM.cppm:290-316),
+ * Omega (src/io/lattice.cppm:169-190). */
+typedef struct {
+    int32_t n1, n2, n3;
+    double lattice[9];         /* rows a1, a2, a3 (bohr) */
+    const double* vloc;        /* [n1*n2*n3] Hartree, index i*n2*n3 + j*n3 + k */
+    int32_t natom;
+    const int32_t* species;    /* [natom] index into species[], grouped */
+    const double* tau;         /* [natom*3] fractional */
+    int32_t nspecies;
+    const effm_species* sp;    /* [nspecies] */
+} effm_cell;
+
+/* Context: device, stream, HBM budget (bytes; 0 = no limit).
+ * Replaces: construction of H_psi::Hamiltonian (src/H_psi/hamiltonian.cppm:119). */
+int effm_ctx_create(effm_ctx** out, int device, uint64_t hbm_budget);
+int effm_ctx_destroy(effm_ctx* ctx);
+/* Order all later work after / on this HIP stream (hipStream_t as void*). */
+int effm_set_stream(effm_ctx* ctx, void* stream);
+const char* effm_last_error(const effm_ctx* ctx);
+/* Device bytes held by the context and the budget. */
+int effm_mem_info(const effm_ctx* ctx, uint64_t* used, uint64_t* budget);
+
+/* Upload the cell.  Replaces Hamiltonian::loadVR/loadATOM/loadNCPP +
+ * finalize() (src/H_psi/hamiltonian.cpp:131-153, 267-392).  Drops all
+ * k-point data. */
+int effm_cell_upload(effm_ctx* ctx, const effm_cell* cell);
+
+/* Upload one k-point's G-sphere.  Replaces GKK::loadKPoint + the
+ * Callable's set_ikpt (src/io/GKK.cppm:511-593,
+ * src/H_psi/hamiltonian_callable.cpp:62-85).  K: [npw*3] Cartesian k+G,
+ * kinetic: [npw] (validated against |K|^2/2 to 1e-10(1+E) as
+ * GKK::validateKineticConsistency, src/io/GKK.cppm:652-671), gidx: [npw*3]
+ * centred integer G, kfrac: fractional k.  Host pointers. */
+int effm_kpt_upload(effm_ctx* ctx, int ikpt, int npw, const double* K,
+                    const double* kinetic, const int32_t* gidx,
+                    const double kfrac[3]);
+/* Number of G vectors of a k-point (Callable::dim, hamiltonian.cppm:184). */
+int effm_kpt_dim(const effm_ctx* ctx, int ikpt);
+int effm_kpt_release(effm_ctx* ctx, int ikpt);
+
+/* H|psi> for nb bands at k-point ikpt.
+ * Replaces Hamiltonian::Callable::operator()(psi, hpsi) and the declared but
+ * undefined batched operator()(n_bands, psi, hpsi)
+ * (src/H_psi/hamiltonian.cppm:177-182; body
+ * src/H_psi/hamiltonian_callable.cpp:94-304).  ld / ld_out in complex
+ * elements (>= npw). */
+int effm_hpsi(effm_ctx* ctx, int ikpt, int nb, const void* psi, int64_t ld,
+              void* hpsi, int64_t ld_out, int prec, int flags);
+
+/* dH/dk_alpha |psi>, alpha = x, y, z; a NULL output skips that direction.
+ * Replaces Hamiltonian::Gradient::Callable::operator()(psi, GradHPsi)
+ * (src/H_psi/hamiltonian.cppm:228-229; body
+ * src/H_psi/hamiltonian_gradient.cpp:116-294). */
+int effm_dhdk(effm_ctx* ctx, int ikpt, int nb, const void* psi, int64_t ld,
+              void* out_x, void* out_y, void* out_z, int64_t ld_out,
+              int prec, int flags);
+
+/* Batched ragged apply: nk k-points in one set of launches, band-major
+ * psi/hpsi blocks of nb bands per k-point at psi_ptrs[i] / hpsi_ptrs[i]
+ * (device pointers), ld[i] >= npw(k).  No reference counterpart (the
+ * reference applies one band at one k); results equal effm_hpsi per k. */
+int effm_hpsi_multi(effm_ctx* ctx, int nk, const int* ikpts, int nb,
+                    const void* const* psi_ptrs, const int64_t* ld,
+                    void* const* hpsi_ptrs, int prec, int flags);
+
+/* FP64 Rayleigh quotients E_b = Re<psi_b|hpsi_b> / <psi_b|psi_b> over npw
+ * elements (test_hpsi_eigen.cpp:49-56 computes this on the host); E is a
+ * device pointer unless EFFM_F_HOST.  prec selects the element type. */
+int effm_rayleigh(effm_ctx* ctx, int npw, int nb, const void* psi, int64_t ld,
+                  const void* hpsi, int64_t ld_h, int prec, double* E, int flags);
+
+/* Per-kernel timing with HIP events on the context's stream (0 = off). */
+int effm_set_timing(effm_ctx* ctx, int enable);
+/* Accumulated milliseconds and launch count of a kernel class; names:
+ * "plane_g2r", "xpencil_vloc", "plane_r2g", "proj_gemm", "proj_reduce",
+ * "proj_apply", "proj_build", "rayleigh". */
+int effm_get_timing(const effm_ctx* ctx, const char* name, double* ms, int64_t* count);
+int effm_reset_timing(effm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

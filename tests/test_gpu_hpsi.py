@@ -1,0 +1,264 @@
+"""Parity of the gfx950 H|psi> / dH/dk path (through the C-ABI) with the
+FP64 oracle.  Tolerances: FP64 device path <= 1e-12 relative per band;
+mixed path energies within 1e-6 Ry (5e-7 Ha) of the FP64 oracle."""
+import copy
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import workload
+from effmass_amd import synth, _lib
+from effmass_amd.hamiltonian import GradHPsi, Hamiltonian
+
+pytestmark = pytest.mark.gpu
+
+RY_TOL_HA = 0.5e-6
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def band_rel(got, ref):
+    num = np.linalg.norm(got - ref, axis=-1)
+    den = np.linalg.norm(ref, axis=-1)
+    return float((num / den).max())
+
+
+def run_h(wl, ik=0, nb=None, prec="fp64", dev=True):
+    torch = _torch()
+    H = Hamiltonian(wl.cell)
+    H.load_kpoint(ik, wl.kpts[ik])
+    psi = wl.psi[ik][:nb] if nb else wl.psi[ik]
+    if prec == "mixed":
+        psi = psi.astype(np.complex64)
+    if dev:
+        p = torch.from_numpy(psi).cuda()
+        h = torch.zeros_like(p)
+        H.at_k(ik)(p, h, prec=prec)
+        torch.cuda.synchronize()
+        out = h.cpu().numpy()
+    else:
+        out = np.zeros_like(psi)
+        H.at_k(ik)(psi, out, prec=prec)
+    H.close()
+    return out
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_config0_fp64():
+    wl = workload(0)
+    got = run_h(wl)
+    ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0])
+    e = band_rel(got, ref)
+    log(f"config0 fp64 rel err {e:.2e}")
+    assert e <= 1e-12
+
+
+def test_two_species_offgamma_fp64():
+    wl = synth.custom(*synth.fcc_primitive(10.26), (36, 36, 36), 30.0,
+                      [np.array([0.2, -0.1, 0.35])], 5, seed=11,
+                      species_idx=[0, 1],
+                      species_list=[synth.make_species(synth.SplitMix64(1)),
+                                    synth.make_species(synth.SplitMix64(2), z=7, ls=(0, 1, 1),
+                                                       widths=(0.3, 0.6))])
+    got = run_h(wl)
+    ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0])
+    assert band_rel(got, ref) <= 1e-12
+
+
+def test_config3_fp64_subset():
+    wl = workload(3, nb=24)
+    got = run_h(wl)
+    ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0], nthreads=8)
+    e = band_rel(got, ref)
+    log(f"config3 fp64 rel err {e:.2e}")
+    assert e <= 1e-12
+
+
+def test_config1_fp64_subset():
+    wl = workload(1, nb=16)
+    got = run_h(wl)
+    ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0], nthreads=8)
+    e = band_rel(got, ref)
+    log(f"config1 fp64 rel err {e:.2e}")
+    assert e <= 1e-12
+
+
+def test_config1_mixed_energies():
+    wl = workload(1, nb=16)
+    got = run_h(wl, prec="mixed")
+    ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0], nthreads=8)
+    psi = wl.psi[0]
+    e_ref = np.real(np.sum(psi.conj() * ref, axis=1)) / np.sum(np.abs(psi) ** 2, axis=1)
+    p32 = psi.astype(np.complex64).astype(np.complex128)
+    e_mix = np.real(np.sum(p32.conj() * got, axis=1)) / np.sum(np.abs(p32) ** 2, axis=1)
+    de = np.abs(e_mix - e_ref).max()
+    log(f"config1 mixed max |dE| {de:.2e} Ha, rel hpsi err {band_rel(got, ref):.2e}")
+    assert de <= RY_TOL_HA
+    assert band_rel(got, ref) <= 2e-5
+
+
+def test_accumulate_overwrite_and_host_path():
+    torch = _torch()
+    wl = workload(0)
+    H = Hamiltonian(wl.cell)
+    H.load_kpoint(0, wl.kpts[0])
+    psi = torch.from_numpy(wl.psi[0]).cuda()
+    base = torch.from_numpy(np.arange(wl.psi[0].size).reshape(wl.psi[0].shape) * (1 - 1j)).cuda()
+    h = base.clone()
+    H.at_k(0)(psi, h)                                   # accumulate (reference semantics)
+    h2 = base.clone()
+    H.at_k(0)(psi, h2, overwrite=True)
+    torch.cuda.synchronize()
+    ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0])
+    assert np.abs(h.cpu().numpy() - base.cpu().numpy() - ref).max() <= 1e-14 * np.abs(base.cpu().numpy()).max()
+    assert band_rel(h2.cpu().numpy(), ref) <= 1e-12
+    hh = np.zeros_like(wl.psi[0])
+    H.at_k(0)(wl.psi[0], hh)                            # host pointers
+    assert band_rel(hh, ref) <= 1e-12
+    # one band at a time equals the batch (Callable(psi, hpsi) on a span)
+    hb = torch.zeros_like(psi[0])
+    H.at_k(0)(psi[3], hb)
+    torch.cuda.synchronize()
+    assert band_rel(hb.cpu().numpy()[None], ref[3:4]) <= 1e-12
+    H.close()
+
+
+def test_dhdk_fp64():
+    torch = _torch()
+    wl = synth.custom(*synth.fcc_primitive(10.26), (36, 36, 36), 30.0,
+                      [np.array([0.1, 0.05, -0.2])], 6, seed=5)
+    H = Hamiltonian(wl.cell)
+    H.load_kpoint(0, wl.kpts[0])
+    psi = torch.from_numpy(wl.psi[0]).cuda()
+    outs = [torch.zeros_like(psi) for _ in range(3)]
+    H.gradient().at_k(0)(psi, GradHPsi(*outs))
+    torch.cuda.synchronize()
+    ref = oracle.dhdk(wl.cell, wl.kpts[0], wl.psi[0])
+    for a in range(3):
+        e = band_rel(outs[a].cpu().numpy(), ref[a])
+        log(f"dhdk dir {a} rel err {e:.2e}")
+        assert e <= 1e-12
+    # skipped direction stays untouched
+    y = torch.full_like(psi, 7.0)
+    x = torch.zeros_like(psi)
+    H.gradient().at_k(0)(psi, GradHPsi(x=x, z=None))
+    torch.cuda.synchronize()
+    assert band_rel(x.cpu().numpy(), ref[0]) <= 1e-12
+    assert (y.cpu().numpy() == 7.0).all()
+    H.close()
+
+
+def test_dhdk_mixed():
+    torch = _torch()
+    wl = workload(0)
+    H = Hamiltonian(wl.cell)
+    H.load_kpoint(0, wl.kpts[0])
+    psi = torch.from_numpy(wl.psi[0].astype(np.complex64)).cuda()
+    outs = [torch.zeros_like(psi) for _ in range(3)]
+    H.gradient().at_k(0)(psi, GradHPsi(*outs), prec="mixed")
+    torch.cuda.synchronize()
+    ref = oracle.dhdk(wl.cell, wl.kpts[0], wl.psi[0])
+    for a in range(3):
+        assert band_rel(outs[a].cpu().numpy(), ref[a]) <= 2e-5
+    H.close()
+
+
+def test_rayleigh_fp64_and_mixed():
+    torch = _torch()
+    wl = workload(0)
+    H = Hamiltonian(wl.cell)
+    H.load_kpoint(0, wl.kpts[0])
+    ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0])
+    psi = wl.psi[0]
+    e_ref = np.real(np.sum(psi.conj() * ref, 1)) / np.sum(np.abs(psi) ** 2, 1)
+    E = H.rayleigh(torch.from_numpy(psi).cuda(), torch.from_numpy(ref).cuda()).cpu().numpy()
+    assert np.abs(E - e_ref).max() <= 1e-13 * np.abs(e_ref).max()
+    E32 = H.rayleigh(psi.astype(np.complex64), ref.astype(np.complex64), prec="mixed")
+    assert np.abs(E32 - e_ref).max() <= 1e-6
+    H.close()
+
+
+def test_error_paths():
+    torch = _torch()
+    wl = workload(0)
+    H = Hamiltonian(wl.cell)
+    with pytest.raises(_lib.EffmOutOfRange):
+        H.at_k(0)                                           # no k-point loaded
+    kp = wl.kpts[0]
+    bad = copy.copy(kp)
+    bad.kin = kp.kin.copy()
+    bad.kin[5] += 1e-6
+    with pytest.raises(_lib.EffmRuntimeError):              # GKK kinetic mismatch
+        H.load_kpoint(0, bad)
+    far = copy.copy(kp)
+    far.K = kp.K * 4.0                                      # q beyond the 20 bohr^-1 table
+    far.kin = 0.5 * (far.K ** 2).sum(1)
+    with pytest.raises(_lib.EffmDomainError):
+        H.load_kpoint(0, far)
+    H.close()
+    with pytest.raises(_lib.EffmOutOfMemory):                # tiny HBM budget
+        Hs = Hamiltonian(wl.cell, hbm_budget=2 << 20)
+        try:
+            Hs.load_kpoint(0, kp)
+            p = torch.from_numpy(wl.psi[0]).cuda()
+            Hs.at_k(0)(p, torch.zeros_like(p))
+        finally:
+            Hs.close()
+
+
+def test_config1_full_hermitian_and_linear():
+    """All 256 bands at the 64-atom size: size-independent properties."""
+    torch = _torch()
+    wl = workload(1)
+    H = Hamiltonian(wl.cell)
+    H.load_kpoint(0, wl.kpts[0])
+    psi = torch.from_numpy(wl.psi[0]).cuda()
+    h = torch.zeros_like(psi)
+    H.at_k(0)(psi, h)
+    M = (psi.conj() @ h.T).cpu().numpy()                     # <psi_m|H|psi_n>
+    herm = np.abs(M - M.conj().T).max() / np.abs(M).max()
+    log(f"config1 hermiticity {herm:.2e}")
+    assert herm <= 1e-12
+    a, b = 0.3 - 0.7j, -1.1 + 0.2j
+    comb = (a * psi[:8] + b * psi[8:16]).contiguous()
+    hc = torch.zeros_like(comb)
+    H.at_k(0)(comb, hc)
+    lin = (hc - (a * h[:8] + b * h[8:16])).abs().max().item() / h[:16].abs().max().item()
+    assert lin <= 1e-12
+    # mixed path energies over all 256 bands within 1e-6 Ry of the FP64 path
+    p32 = psi.to(torch.complex64)
+    h32 = torch.zeros_like(p32)
+    H.at_k(0)(p32, h32, prec="mixed")
+    E64 = H.rayleigh(psi, h).cpu().numpy()
+    E32 = H.rayleigh(p32, h32, prec="mixed").cpu().numpy()
+    de = np.abs(E32 - E64).max()
+    log(f"config1 all-band mixed |dE| max {de:.2e} Ha")
+    assert de <= RY_TOL_HA
+    H.close()
+
+
+def test_multi_k_ragged():
+    torch = _torch()
+    wl = workload(2, nb=4, nk=3)
+    H = Hamiltonian(wl.cell)
+    for ik, kp in enumerate(wl.kpts):
+        H.load_kpoint(ik, kp)
+    psis = [torch.from_numpy(p).cuda() for p in wl.psi]
+    outs = [torch.zeros_like(p) for p in psis]
+    H.apply_multi([0, 1, 2], psis, outs)
+    torch.cuda.synchronize()
+    assert len({kp.npw for kp in wl.kpts}) > 1                # ragged npw
+    for ik in range(3):
+        ref = oracle.hpsi(wl.cell, wl.kpts[ik], wl.psi[ik], nthreads=8)
+        assert band_rel(outs[ik].cpu().numpy(), ref) <= 1e-12
+    H.close()

@@ -1,0 +1,125 @@
+"""Pins the oracle's H|psi> and dH/dk|psi> (src/H_psi/hamiltonian_callable.cpp,
+src/H_psi/hamiltonian_gradient.cpp) against dense matrices built from the
+mathematics on the 2-atom fcc cell (npw = 725)."""
+import copy
+
+import numpy as np
+import pytest
+
+import oracle
+import refmath
+from conftest import workload
+from effmass_amd import synth
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def test_npw_of_configs():
+    assert workload(0, nb=2).kpts[0].npw == 725
+    assert workload(3, nb=2).kpts[0].npw == 6451
+
+
+def test_local_part_vs_dense():
+    wl = workload(0)
+    cell = copy.copy(wl.cell)
+    sp = copy.copy(cell.species_list[0]); sp.lam = np.zeros_like(sp.lam)
+    cell.species_list = [sp]                      # |lambda| < 1e-15: no projectors
+    kp, psi = wl.kpts[0], wl.psi[0]
+    H = refmath.dense_h(cell, kp, nonlocal_=False)
+    got = oracle.hpsi(cell, kp, psi)
+    assert _rel(got, psi @ H.T) < 1e-12
+
+
+def test_full_vs_dense():
+    wl = workload(0)
+    kp, psi = wl.kpts[0], wl.psi[0]
+    H = refmath.dense_h(wl.cell, kp)
+    got = oracle.hpsi(wl.cell, kp, psi)
+    # closed-form beta vs the 4-point table: interpolation error ~1e-9
+    assert _rel(got, psi @ H.T) < 1e-8
+    vnl = oracle.hpsi(wl.cell, kp, psi, nonlocal_only=True)
+    Hn = refmath.dense_h(wl.cell, kp, local=False)
+    assert _rel(vnl, psi @ Hn.T) < 1e-8
+
+
+def test_two_species_vs_dense():
+    wl = workload(0, two_species=True)
+    kp, psi = wl.kpts[0], wl.psi[0]
+    H = refmath.dense_h(wl.cell, kp)
+    assert _rel(oracle.hpsi(wl.cell, kp, psi), psi @ H.T) < 1e-8
+
+
+def test_offgamma_vs_dense():
+    wl = synth.custom(*synth.fcc_primitive(10.26), (36, 36, 36), 30.0,
+                      [np.array([0.2, -0.1, 0.35])], 3, seed=11)
+    kp, psi = wl.kpts[0], wl.psi[0]
+    H = refmath.dense_h(wl.cell, kp)
+    assert _rel(oracle.hpsi(wl.cell, kp, psi), psi @ H.T) < 1e-8
+
+
+def test_hermitian_and_accumulate():
+    wl = workload(0)
+    kp, psi = wl.kpts[0], wl.psi[0]
+    h = oracle.hpsi(wl.cell, kp, psi)
+    M = psi.conj() @ h.T                        # <psi_m|H|psi_n>
+    assert np.abs(M - M.conj().T).max() < 1e-13 * np.abs(M).max()
+    base = np.arange(psi.size).reshape(psi.shape) * (1 + 2j)
+    assert np.abs(oracle.hpsi(wl.cell, kp, psi, base) - (base + h)).max() <= 1e-14 * np.abs(base).max()   # ~20 roundings at ulp(base)
+
+
+def test_constant_potential_shift():
+    wl = workload(0)
+    cell = copy.copy(wl.cell)
+    cell.vloc = np.full_like(cell.vloc, -0.3)
+    sp = copy.copy(cell.species_list[0]); sp.lam = np.zeros_like(sp.lam)
+    cell.species_list = [sp]
+    kp, psi = wl.kpts[0], wl.psi[0]
+    got = oracle.hpsi(cell, kp, psi)
+    assert np.abs(got - (kp.kin - 0.3) * psi).max() < 1e-14
+
+
+def _shift_k(kp, cell, alpha, d):
+    K = kp.K.copy(); K[:, alpha] += d
+    dk = np.zeros(3); dk[alpha] = d
+    kf = kp.kfrac + cell.A @ dk / (2 * np.pi)
+    return synth.KPoint(kfrac=kf, K=K, kin=0.5 * (K * K).sum(1), gidx=kp.gidx)
+
+
+def test_dhdk_vs_finite_difference():
+    wl = synth.custom(*synth.fcc_primitive(10.26), (36, 36, 36), 30.0,
+                      [np.array([0.1, 0.05, -0.2])], 3, seed=5)
+    cell, kp, psi = wl.cell, wl.kpts[0], wl.psi[0]
+    rng = np.random.default_rng(0)
+    phi = (rng.standard_normal(psi.shape) + 1j * rng.standard_normal(psi.shape)) / (1 + kp.kin)
+    g = oracle.dhdk(cell, kp, psi)
+    d = 1e-4
+    for al in range(3):
+        hp = oracle.hpsi(cell, _shift_k(kp, cell, al, d), psi)
+        hm = oracle.hpsi(cell, _shift_k(kp, cell, al, -d), psi)
+        fd = phi.conj() @ ((hp - hm) / (2 * d)).T
+        an = phi.conj() @ g[al].T
+        assert np.abs(fd - an).max() < 1e-6 * np.abs(an).max(), al
+
+
+def test_dhdk_vs_dense_derivative():
+    wl = workload(0)
+    cell, kp, psi = wl.cell, wl.kpts[0], wl.psi[0]
+    P, lam, dP = refmath.projectors(cell, kp, deriv=True)
+    g = oracle.dhdk(cell, kp, psi)
+    for al in range(3):
+        D = np.diag(kp.K[:, al]) + (dP[:, al].T * lam) @ P.conj() + (P.T * lam) @ dP[:, al].conj()
+        ref = psi @ D.T
+        assert _rel(g[al], ref) < 1e-7, al
+
+
+def test_dhdk_kinetic_only_and_skip():
+    wl = workload(0)
+    cell = copy.copy(wl.cell)
+    sp = copy.copy(cell.species_list[0]); sp.lam = np.zeros_like(sp.lam)
+    cell.species_list = [sp]
+    kp, psi = wl.kpts[0], wl.psi[0]
+    x, y, z = oracle.dhdk(cell, kp, psi, dirs=(True, False, True))
+    assert y is None
+    assert np.abs(x - kp.K[:, 0] * psi).max() == 0 and np.abs(z - kp.K[:, 2] * psi).max() == 0
