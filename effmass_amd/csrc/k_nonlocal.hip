@@ -35,6 +35,16 @@ __device__ __forceinline__ v4f mfma16(float a, float b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// Row of the 16x16 output block held in accumulator register v of a lane in
+// lane group kk = lane / 16 (the column is lane % 16).  A and B operands
+// share one layout (row or column = lane % 16, k = lane / 16), but the
+// outputs differ: f32 16x16x4 keeps 4 consecutive rows per lane group,
+// f64 16x16x4 interleaves them with stride 4.
+template <typename T>
+__device__ __forceinline__ int out_row(int kk, int v) {
+    return sizeof(T) == 8 ? v * 4 + kk : kk * 4 + v;
+}
+
 // ---------------------------------------------------------------------------
 //  Projector generation
 // ---------------------------------------------------------------------------
@@ -280,7 +290,7 @@ k_gemm_proj(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restric
     for (int r = 0; r < 2; ++r)
         for (int q = 0; q < 2; ++q)
             for (int v = 0; v < 4; ++v) {
-                int row = m0 + r * 16 + kk * 4 + v;
+                int row = m0 + r * 16 + out_row<T>(kk, v);
                 int col = n0 + q * 16 + i;
                 if (row < M && col < N)
                     po[(int64_t)(row0 + row) * N + col] = C{accR[r][q][v], accI[r][q][v]};
@@ -353,7 +363,7 @@ k_gemm_apply(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restri
             const int n = n0 + q * 16 + i;
             if (n >= N) continue;
             for (int v = 0; v < 4; ++v) {
-                const int m = m0 + r * 16 + kk * 4 + v;
+                const int m = m0 + r * 16 + out_row<T>(kk, v);
                 if (m >= M) continue;
                 double re = (double)accR[r][q][v], im = (double)accI[r][q][v];
                 if (diag) {

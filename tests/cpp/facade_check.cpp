@@ -1,0 +1,105 @@
+// Exercises the C++ facade (effmass_amd/host/effm_hamiltonian.hpp) the way the
+// reference's tests drive H_psi::Hamiltonian: at_k(ikpt)(psi, hpsi) with
+// accumulate semantics, gradient().at_k(ikpt)(psi, GradHPsi{...}) with a
+// skipped direction, and the exception classes of the error paths.
+// Closed-form case: simple cubic cell, constant V, one s projector
+// beta(q) = exp(-q^2/4) on one atom, so H psi is known analytically.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <stdexcept>
+#include <vector>
+
+#include "effm_hamiltonian.hpp"
+
+using cd = std::complex<double>;
+
+int main() {
+    const double a = 10.0, V0 = 0.1, lam = 0.5, PI = 3.14159265358979323846;
+    const int n = 24;
+    effm::CellData cell;
+    cell.n1 = cell.n2 = cell.n3 = n;
+    cell.lattice[0] = cell.lattice[4] = cell.lattice[8] = a;
+    cell.vloc.assign((size_t)n * n * n, V0);
+    cell.species = {0};
+    cell.tau = {0.1, 0.2, 0.3};
+    effm::SpeciesData sp;
+    sp.lmax = 0;
+    sp.beta_l = {0};
+    sp.lambda = {lam};
+    sp.nq = 2004;
+    for (int i = 0; i < sp.nq; ++i) {
+        double q = i * sp.dq;
+        sp.beta_tab.push_back(std::exp(-q * q / 4));
+        sp.dbeta_tab.push_back(-0.5 * q * std::exp(-q * q / 4));
+    }
+    cell.sp = {sp};
+    effm::KPointData k;
+    const double b = 2 * PI / a, ecut_ry = 10.0;
+    for (int i = -n / 2; i < n / 2; ++i)
+        for (int j = -n / 2; j < n / 2; ++j)
+            for (int l = -n / 2; l < n / 2; ++l) {
+                double K[3] = {i * b, j * b, l * b};
+                double k2 = K[0] * K[0] + K[1] * K[1] + K[2] * K[2];
+                if (k2 > ecut_ry) continue;
+                k.K.insert(k.K.end(), K, K + 3);
+                k.kinetic.push_back(0.5 * k2);
+                k.gidx.insert(k.gidx.end(), {i, j, l});
+            }
+    const int npw = k.npw();
+    effm::Hamiltonian h(cell);
+    h.loadKPoint(0, k);
+    std::vector<cd> psi(npw), hpsi(npw, cd(1.0, -1.0)), p(npw), dp[3];
+    for (auto& v : dp) v.assign(npw, 0.0);
+    const double omega = a * a * a, y00 = 0.5 / std::sqrt(PI);
+    cd inner = 0, dinner[3] = {0, 0, 0};
+    for (int g = 0; g < npw; ++g) {
+        psi[g] = cd(std::cos(0.3 * g), std::sin(0.17 * g)) / (1.0 + k.kinetic[g]);
+        const double* K = &k.K[3 * g];
+        double q = std::sqrt(K[0] * K[0] + K[1] * K[1] + K[2] * K[2]);
+        double ph = -2 * PI * (k.gidx[3 * g] * 0.1 + k.gidx[3 * g + 1] * 0.2 + k.gidx[3 * g + 2] * 0.3);
+        cd S(std::cos(ph), std::sin(ph));
+        p[g] = std::exp(-q * q / 4) / std::sqrt(omega) * y00 * S;
+        inner += std::conj(p[g]) * psi[g];
+        for (int d = 0; d < 3; ++d) {
+            dp[d][g] = (-0.5 * q * std::exp(-q * q / 4)) * (q > 0 ? K[d] / q : 0.0) / std::sqrt(omega) * y00 * S;
+            dinner[d] += std::conj(dp[d][g]) * psi[g];
+        }
+    }
+    auto op = h.at_k(0);
+    op(psi, hpsi);                                     // accumulate onto (1 - i)
+    double err = 0, ref = 0;
+    for (int g = 0; g < npw; ++g) {
+        cd e = cd(1.0, -1.0) + (k.kinetic[g] + V0) * psi[g] + lam * inner * p[g];
+        err = std::max(err, std::abs(hpsi[g] - e));
+        ref = std::max(ref, std::abs(e));
+    }
+    std::printf("facade H|psi> max err %.3e (npw %d)\n", err, npw);
+    if (err > 1e-9 * ref) return 1;
+    std::vector<cd> gx(npw), gz(npw);
+    h.gradient().at_k(0)(psi, effm::GradHPsi{gx, {}, gz});
+    double gerr = 0;
+    for (int g = 0; g < npw; ++g) {
+        for (int d : {0, 2}) {
+            cd e = k.K[3 * g + d] * psi[g] + lam * (dinner[d] * p[g] + inner * dp[d][g]);
+            gerr = std::max(gerr, std::abs((d == 0 ? gx : gz)[g] - e));
+        }
+    }
+    std::printf("facade dH/dk max err %.3e\n", gerr);
+    if (gerr > 1e-8) return 2;
+    try {
+        h.at_k(7);
+        return 3;
+    } catch (const std::out_of_range&) {
+    }
+    try {
+        effm::KPointData far = k;
+        for (auto& v : far.K) v *= 8.0;
+        for (int g = 0; g < npw; ++g) far.kinetic[g] *= 64.0;
+        h.loadKPoint(1, far);
+        return 4;
+    } catch (const std::domain_error&) {
+    }
+    std::printf("facade checks passed\n");
+    return 0;
+}
