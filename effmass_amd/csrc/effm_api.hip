@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -79,26 +80,32 @@ static int upload(Ctx& c, DevBuf& b, const std::vector<V>& h, const char* what) 
     return kOk;
 }
 
-// Radix sequence for an axis: 4s, then 2, 3, 5, then remaining primes.
+// Radix sequence for an axis: fewest stages (each stage is one pass over
+// LDS), radices from the in-register DFTs of fft_lds.hpp; ties go to the
+// sequence whose largest radix is smallest (fewer registers).
 static bool factor_axis(int n, FftAxis& ax) {
+    static const int radices[] = {12, 10, 9, 8, 7, 6, 5, 4, 3, 2};
     ax.n = n;
     ax.ns = 0;
-    int m = n;
-    const int pref[] = {4, 2, 3, 5};
-    for (int p : pref)
-        while (m % p == 0 && m > 1) {
-            if (ax.ns >= kMaxStages) return false;
-            ax.p[ax.ns++] = p;
-            m /= p;
+    if (n == 1) return true;
+    std::vector<int> best(n + 1, 1 << 20), pick(n + 1, 0), worst(n + 1, 0);
+    best[1] = 0;
+    for (int m = 2; m <= n; ++m) {
+        if (n % m) continue;
+        for (int p : radices) {
+            if (m % p || best[m / p] >= (1 << 20)) continue;
+            int st = best[m / p] + 1;
+            int wr = std::max(worst[m / p], p);
+            if (st < best[m] || (st == best[m] && wr < worst[m])) {
+                best[m] = st;
+                pick[m] = p;
+                worst[m] = wr;
+            }
         }
-    const int more[] = {7, 11, 13};
-    for (int p : more)
-        while (m % p == 0) {
-            if (ax.ns >= kMaxStages) return false;
-            ax.p[ax.ns++] = p;
-            m /= p;
-        }
-    return m == 1;
+    }
+    if (best[n] >= (1 << 20) || best[n] > kMaxStages) return false;
+    for (int m = n; m > 1; m /= pick[m]) ax.p[ax.ns++] = pick[m];
+    return true;
 }
 
 // perm(j): position of input j for the in-place DIT (= position of output
@@ -233,7 +240,9 @@ static int run_hpsi(Ctx& c, KptDev& k, int nb, const void* psi, int64_t ld, void
     int rc = ensure_proj<T>(c, k, false);
     if (rc) return rc;
     const size_t wpb = work_per_band<T>(c, k);
-    int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)nb, kWorkCap / std::max<size_t>(wpb, 1)));
+    size_t cap = kWorkCap;
+    if (const char* e = std::getenv("EFFM_WORK_MB")) cap = std::max<size_t>(1, std::strtoull(e, nullptr, 10)) << 20;
+    int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)nb, cap / std::max<size_t>(wpb, 1)));
     if ((rc = ensure(c, c.work, wpb * chunk, "FFT workspace"))) return rc;
     for (int b0 = 0; b0 < nb; b0 += chunk) {
         int nbc = std::min(chunk, nb - b0);
@@ -336,7 +345,7 @@ int effm_cell_upload(effm_ctx* h, const effm_cell* cell) {
         return fail(c, kErrRuntime, "H|psi> requires ATOM (species / tau missing)");
     FftAxis a1, a2, a3;
     if (!factor_axis(cell->n1, a1) || !factor_axis(cell->n2, a2) || !factor_axis(cell->n3, a3))
-        return fail(c, kErrInvalid, "FFT grid sizes must factor into 2, 3, 5, 7, 11, 13");
+        return fail(c, kErrInvalid, "FFT grid sizes must factor into 2, 3, 5 and 7");
     size_t plane64 = sizeof(double2) * ((size_t)cell->n2 * cell->n3 + cell->n2 + cell->n3);
     if (plane64 > 160 * 1024)
         return fail(c, kErrInvalid, "n2*n3 plane exceeds the 160 KiB LDS of a gfx950 CU");

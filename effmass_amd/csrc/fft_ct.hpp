@@ -1,0 +1,79 @@
+// fft_ct.hpp — compile-time two-stage plans for the common grid axes.
+//
+// Same in-place algorithm and digit-reversed order as fft_lds.hpp (so the
+// host-side permutations are shared), but the length N = P1 * P2, the line
+// count and the workgroup size are template parameters: every index
+// division folds to a multiply/shift, the radix dispatch disappears and the
+// per-thread loop is fully unrolled.  A two-stage plan is one LDS round trip
+// per radix, i.e. two per axis.
+#pragma once
+#include "fft_lds.hpp"
+
+namespace effm {
+
+// Radix pair (first DIT stage P1, second P2) of the specialised axes.  Must
+// match factor_axis() on the host for these sizes.
+template <int N> struct Plan2 { static constexpr bool ok = false; static constexpr int P1 = 1, P2 = 1; };
+template <> struct Plan2<90> { static constexpr bool ok = true; static constexpr int P1 = 10, P2 = 9; };
+template <> struct Plan2<48> { static constexpr bool ok = true; static constexpr int P1 = 8, P2 = 6; };
+template <> struct Plan2<36> { static constexpr bool ok = true; static constexpr int P1 = 6, P2 = 6; };
+
+// One radix-P stage (combining blocks of L) over NL lines of an N-point axis;
+// element e of line l at l*LS + e*ES.  Lines vary fastest across threads when
+// the axis is strided (ES > 1), butterflies otherwise.
+template <int N, int P, int L, int S, bool DIT, int ES, int LS, int NL, int BLOCK, typename T>
+__device__ __forceinline__ void ct_stage(cx_t<T>* buf, const cx_t<T>* tw) {
+    constexpr int NBF = N / P;
+    constexpr int TOTAL = NL * NBF;
+    constexpr int LT = L * P;
+    constexpr int TSTEP = N / LT;
+    constexpr int ITER = (TOTAL + BLOCK - 1) / BLOCK;
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+        const int t = threadIdx.x + it * BLOCK;
+        if (ITER * BLOCK == TOTAL || t < TOTAL) {
+            int line, r;
+            if constexpr (ES != 1) {
+                r = t / NL;
+                line = t - r * NL;
+            } else {
+                line = t / NBF;
+                r = t - line * NBF;
+            }
+            const int b = r / L;
+            const int k1 = r - b * L;
+            const int base = line * LS + (b * LT + k1) * ES;
+            butterfly<P, S, DIT, T>(buf, base, L * ES, k1, TSTEP, tw);
+        }
+    }
+}
+
+// Full N-point transform along one axis; ends with a barrier.
+template <int N, int S, bool DIT, int ES, int LS, int NL, int BLOCK, typename T>
+__device__ __forceinline__ void ct_fft(cx_t<T>* buf, const cx_t<T>* tw) {
+    constexpr int P1 = Plan2<N>::P1, P2 = Plan2<N>::P2;
+    if constexpr (DIT) {
+        ct_stage<N, P1, 1, S, true, ES, LS, NL, BLOCK, T>(buf, tw);
+        __syncthreads();
+        ct_stage<N, P2, P1, S, true, ES, LS, NL, BLOCK, T>(buf, tw);
+        __syncthreads();
+    } else {
+        ct_stage<N, P2, P1, S, false, ES, LS, NL, BLOCK, T>(buf, tw);
+        __syncthreads();
+        ct_stage<N, P1, 1, S, false, ES, LS, NL, BLOCK, T>(buf, tw);
+        __syncthreads();
+    }
+}
+
+constexpr int round_up(int a, int b) { return (a + b - 1) / b * b; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int cmin(int a, int b) { return a < b ? a : b; }
+
+// Workgroup size for a square N x N plane: one butterfly per thread and
+// stage, rounded to whole waves, at most 1024.
+template <int N>
+constexpr int plane_block() {
+    return cmin(1024, round_up(N * cmax(N / Plan2<N>::P1, N / Plan2<N>::P2), 64));
+}
+
+}  // namespace effm

@@ -113,28 +113,130 @@ __device__ __forceinline__ void dft5(cx_t<T>* y) {
     y[3] = csub(m2, r2);
 }
 
-// Other prime radices (7, 11, 13) by direct summation with table twiddles
-// W_p^{uq} = W_n^{(uq mod p) n/p}; P is a compile-time constant so the
-// working set stays in registers.
+// ---- compile-time twiddles ------------------------------------------------
+// cos / sin of 2 pi k / P evaluated at compile time: the angle is reduced to
+// an octant of |x| <= pi/4 and summed as a Taylor series to full double
+// precision.
+namespace ce {
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double taylor_sin(double x) {
+    double t = x, s = x;
+    for (int k = 1; k < 14; ++k) { t *= -x * x / ((2 * k) * (2 * k + 1)); s += t; }
+    return s;
+}
+constexpr double taylor_cos(double x) {
+    double t = 1.0, s = 1.0;
+    for (int k = 1; k < 14; ++k) { t *= -x * x / ((2 * k - 1) * (2 * k)); s += t; }
+    return s;
+}
+// cos(2 pi k / P) and sin(2 pi k / P) via the octant of 8k/P
+constexpr double cos2pi(int k, int P) {
+    k = ((k % P) + P) % P;
+    // angle = 2 pi k / P = (pi/4) * (8k/P); octant o = floor(8k/P), rem r
+    int o = (8 * k) / P;
+    double r = (kPi / 4) * ((double)(8 * k - o * P) / P);   // in [0, pi/4)
+    switch (o) {
+        case 0: return taylor_cos(r);
+        case 1: return taylor_sin(kPi / 4 - r);
+        case 2: return -taylor_sin(r);
+        case 3: return -taylor_cos(kPi / 4 - r);
+        case 4: return -taylor_cos(r);
+        case 5: return -taylor_sin(kPi / 4 - r);
+        case 6: return taylor_sin(r);
+        default: return taylor_cos(kPi / 4 - r);
+    }
+}
+constexpr double sin2pi(int k, int P) { return cos2pi(k * 4 - P, 4 * P); }
+}  // namespace ce
+
+// y_q <- sum_u y_u exp(S 2 pi i u q / P) for any P used by the plans.
 template <int P, int S, typename T>
-__device__ __forceinline__ void dftp(cx_t<T>* y, const cx_t<T>* tw, int n) {
+__device__ __forceinline__ void dft(cx_t<T>* y);
+
+// Radix 7 by the symmetric pairs (y_u +- y_{7-u}), like dft5.
+template <int S, typename T>
+__device__ __forceinline__ void dft7(cx_t<T>* y) {
     using C = cx_t<T>;
-    C out[P];
-    const int st = n / P;
+    const T c1 = T(0.62348980185873353052500488400424);    // cos(2 pi / 7)
+    const T c2 = T(-0.22252093395631440428890256449679);   // cos(4 pi / 7)
+    const T c3 = T(-0.90096886790241912623610231950745);   // cos(6 pi / 7)
+    const T s1 = T(0.78183148246802980870844452667406);    // sin(2 pi / 7)
+    const T s2 = T(0.97492791218182360701813168299393);    // sin(4 pi / 7)
+    const T s3 = T(0.43388373911755812047576833284836);    // sin(6 pi / 7)
+    C t1 = cadd(y[1], y[6]), d1 = csub(y[1], y[6]);
+    C t2 = cadd(y[2], y[5]), d2 = csub(y[2], y[5]);
+    C t3 = cadd(y[3], y[4]), d3 = csub(y[3], y[4]);
+    C m1 = C{y[0].x + c1 * t1.x + c2 * t2.x + c3 * t3.x, y[0].y + c1 * t1.y + c2 * t2.y + c3 * t3.y};
+    C m2 = C{y[0].x + c2 * t1.x + c3 * t2.x + c1 * t3.x, y[0].y + c2 * t1.y + c3 * t2.y + c1 * t3.y};
+    C m3 = C{y[0].x + c3 * t1.x + c1 * t2.x + c2 * t3.x, y[0].y + c3 * t1.y + c1 * t2.y + c2 * t3.y};
+    // sin(2 pi u q / 7) for q = 1, 2, 3 over u = 1, 2, 3
+    C r1 = cmul_si<S>(C{s1 * d1.x + s2 * d2.x + s3 * d3.x, s1 * d1.y + s2 * d2.y + s3 * d3.y});
+    C r2 = cmul_si<S>(C{s2 * d1.x - s3 * d2.x - s1 * d3.x, s2 * d1.y - s3 * d2.y - s1 * d3.y});
+    C r3 = cmul_si<S>(C{s3 * d1.x - s1 * d2.x + s2 * d3.x, s3 * d1.y - s1 * d2.y + s2 * d3.y});
+    y[0] = cadd(y[0], cadd(t1, cadd(t2, t3)));
+    y[1] = cadd(m1, r1);
+    y[6] = csub(m1, r1);
+    y[2] = cadd(m2, r2);
+    y[5] = csub(m2, r2);
+    y[3] = cadd(m3, r3);
+    y[4] = csub(m3, r3);
+}
+
+// Composite P = P1 * P2 by one Cooley-Tukey split in registers:
+// u = P2 u1 + u2, q = q1 + P1 q2:
+//   Z[u2][q1] = DFT_P1 over u1 of y[P2 u1 + u2], times w^(u2 q1),
+//   X[q1 + P1 q2] = DFT_P2 over u2 of Z[u2][q1].
+template <int P1, int P2, int S, typename T>
+__device__ __forceinline__ void dft_split(cx_t<T>* y) {
+    using C = cx_t<T>;
+    constexpr int P = P1 * P2;
+    C z[P];
 #pragma unroll
-    for (int q = 0; q < P; ++q) {
-        C acc = y[0];
+    for (int u2 = 0; u2 < P2; ++u2) {
+        C a[P1];
 #pragma unroll
-        for (int u = 1; u < P; ++u) acc = cadd(acc, cmul(y[u], twid<S>(tw, ((u * q) % P) * st)));
-        out[q] = acc;
+        for (int u1 = 0; u1 < P1; ++u1) a[u1] = y[P2 * u1 + u2];
+        dft<P1, S, T>(a);
+#pragma unroll
+        for (int q1 = 0; q1 < P1; ++q1) {
+            if (u2 * q1 != 0) {
+                const C w = C{(T)ce::cos2pi(u2 * q1, P), (T)(S * ce::sin2pi(u2 * q1, P))};
+                a[q1] = cmul(a[q1], w);
+            }
+            z[u2 * P1 + q1] = a[q1];
+        }
     }
 #pragma unroll
-    for (int q = 0; q < P; ++q) y[q] = out[q];
+    for (int q1 = 0; q1 < P1; ++q1) {
+        C b[P2];
+#pragma unroll
+        for (int u2 = 0; u2 < P2; ++u2) b[u2] = z[u2 * P1 + q1];
+        dft<P2, S, T>(b);
+#pragma unroll
+        for (int q2 = 0; q2 < P2; ++q2) y[q1 + P1 * q2] = b[q2];
+    }
+}
+
+template <int P, int S, typename T>
+__device__ __forceinline__ void dft(cx_t<T>* y) {
+    if constexpr (P == 1) {
+    } else if constexpr (P == 2) dft2<S, T>(y);
+    else if constexpr (P == 3) dft3<S, T>(y);
+    else if constexpr (P == 4) dft4<S, T>(y);
+    else if constexpr (P == 5) dft5<S, T>(y);
+    else if constexpr (P == 6) dft_split<2, 3, S, T>(y);
+    else if constexpr (P == 8) dft_split<2, 4, S, T>(y);
+    else if constexpr (P == 9) dft_split<3, 3, S, T>(y);
+    else if constexpr (P == 10) dft_split<2, 5, S, T>(y);
+    else if constexpr (P == 12) dft_split<3, 4, S, T>(y);
+    else if constexpr (P == 15) dft_split<3, 5, S, T>(y);
+    else if constexpr (P == 16) dft_split<4, 4, S, T>(y);
+    else if constexpr (P == 7) dft7<S, T>(y);
 }
 
 template <int P, int S, bool DIT, typename T>
 __device__ __forceinline__ void butterfly(cx_t<T>* buf, int base, int es, int k1, int tstep,
-                                          const cx_t<T>* tw, int n) {
+                                          const cx_t<T>* tw) {
     using C = cx_t<T>;
     C y[P];
 #pragma unroll
@@ -143,11 +245,7 @@ __device__ __forceinline__ void butterfly(cx_t<T>* buf, int base, int es, int k1
 #pragma unroll
         for (int u = 1; u < P; ++u) y[u] = cmul(y[u], twid<S>(tw, u * k1 * tstep));
     }
-    if constexpr (P == 2) dft2<S, T>(y);
-    else if constexpr (P == 3) dft3<S, T>(y);
-    else if constexpr (P == 4) dft4<S, T>(y);
-    else if constexpr (P == 5) dft5<S, T>(y);
-    else dftp<P, S, T>(y, tw, n);
+    dft<P, S, T>(y);
     if (!DIT && k1 != 0) {
 #pragma unroll
         for (int u = 1; u < P; ++u) y[u] = cmul(y[u], twid<S>(tw, u * k1 * tstep));
@@ -166,21 +264,29 @@ __device__ void fft_stage(cx_t<T>* buf, int n, int p, int L, int es, int ls, int
     const int total = nlines * nbf;
     const int Lt = L * p;
     const int tstep = n / Lt;
+    // Strided axes (es > 1, lines adjacent) put consecutive lanes on
+    // consecutive lines so a wave's 16-byte LDS accesses are contiguous;
+    // the contiguous axis keeps butterflies of one line on adjacent lanes.
+    const bool line_fast = es != 1;
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
-        int line = t / nbf;
-        int r = t - line * nbf;
+        int line, r;
+        if (line_fast) {
+            r = t / nlines;
+            line = t - r * nlines;
+        } else {
+            line = t / nbf;
+            r = t - line * nbf;
+        }
         int b = r / L;
         int k1 = r - b * L;
         int base = line * ls + (b * Lt + k1) * es;
         int ues = L * es;
         switch (p) {
-            case 2: butterfly<2, S, DIT, T>(buf, base, ues, k1, tstep, tw, n); break;
-            case 3: butterfly<3, S, DIT, T>(buf, base, ues, k1, tstep, tw, n); break;
-            case 4: butterfly<4, S, DIT, T>(buf, base, ues, k1, tstep, tw, n); break;
-            case 5: butterfly<5, S, DIT, T>(buf, base, ues, k1, tstep, tw, n); break;
-            case 7: butterfly<7, S, DIT, T>(buf, base, ues, k1, tstep, tw, n); break;
-            case 11: butterfly<11, S, DIT, T>(buf, base, ues, k1, tstep, tw, n); break;
-            default: butterfly<13, S, DIT, T>(buf, base, ues, k1, tstep, tw, n); break;
+#define EFFM_RADIX(P) case P: butterfly<P, S, DIT, T>(buf, base, ues, k1, tstep, tw); break;
+            EFFM_RADIX(2) EFFM_RADIX(3) EFFM_RADIX(4) EFFM_RADIX(5) EFFM_RADIX(6)
+            EFFM_RADIX(7) EFFM_RADIX(8) EFFM_RADIX(9) EFFM_RADIX(10) EFFM_RADIX(12)
+#undef EFFM_RADIX
+            default: break;
         }
     }
 }

@@ -19,6 +19,7 @@
 // Planes outside the sphere's x-range are never written or read: this is
 // the zero-padding the reference's dense 3D FFT spends on them.
 #include "effm_internal.hpp"
+#include "fft_ct.hpp"
 
 namespace effm {
 
@@ -138,6 +139,211 @@ k_plane_r2g(const cx_t<T>* __restrict__ work, int nplanes, const int* __restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+//  Specialised kernels for square n2 = n3 = N planes and N1-point x pencils
+//  with compile-time two-stage plans (fft_ct.hpp).  Same data flow as above.
+// ---------------------------------------------------------------------------
+template <typename T, int N, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+k_plane_g2r_ct(const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict__ work, int nplanes,
+               const int* __restrict__ plane_ptr, const int* __restrict__ ent_ig,
+               const int* __restrict__ ent_pos, const cx_t<T>* __restrict__ tw_g) {
+    using C = cx_t<T>;
+    constexpr int N2 = N * N;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* plane = reinterpret_cast<C*>(smem_raw);
+    C* tw = plane + N2;
+    const int p = blockIdx.x;
+    const int b = blockIdx.y;
+#pragma unroll
+    for (int i = threadIdx.x; i < N2; i += BLOCK) plane[i] = C{0, 0};
+    if (threadIdx.x < N) tw[threadIdx.x] = tw_g[threadIdx.x];
+    __syncthreads();
+    const C* ps = psi + (int64_t)b * ld;
+    const int e0 = plane_ptr[p], e1 = plane_ptr[p + 1];
+    for (int e = e0 + threadIdx.x; e < e1; e += BLOCK) plane[ent_pos[e]] = ps[ent_ig[e]];
+    __syncthreads();
+    ct_fft<N, +1, true, 1, N, N, BLOCK, T>(plane, tw);     // along k (rows)
+    ct_fft<N, +1, true, N, 1, N, BLOCK, T>(plane, tw);     // along j (columns)
+    C* out = work + ((int64_t)b * nplanes + p) * N2;
+#pragma unroll
+    for (int i = threadIdx.x; i < N2; i += BLOCK) out[i] = plane[i];
+}
+
+template <typename T, int N, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+k_plane_r2g_ct(const cx_t<T>* __restrict__ work, int nplanes, const int* __restrict__ plane_ptr,
+               const int* __restrict__ ent_ig, const int* __restrict__ ent_pos,
+               const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict__ hpsi, int64_t ldo,
+               const double* __restrict__ kin, T scale, int accumulate,
+               const cx_t<T>* __restrict__ tw_g) {
+    using C = cx_t<T>;
+    constexpr int N2 = N * N;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* plane = reinterpret_cast<C*>(smem_raw);
+    C* tw = plane + N2;
+    const int p = blockIdx.x;
+    const int b = blockIdx.y;
+    const C* in = work + ((int64_t)b * nplanes + p) * N2;
+#pragma unroll
+    for (int i = threadIdx.x; i < N2; i += BLOCK) plane[i] = in[i];
+    if (threadIdx.x < N) tw[threadIdx.x] = tw_g[threadIdx.x];
+    __syncthreads();
+    ct_fft<N, -1, false, N, 1, N, BLOCK, T>(plane, tw);    // along j
+    ct_fft<N, -1, false, 1, N, N, BLOCK, T>(plane, tw);    // along k
+    const C* ps = psi + (int64_t)b * ld;
+    C* hs = hpsi + (int64_t)b * ldo;
+    const int e0 = plane_ptr[p], e1 = plane_ptr[p + 1];
+    for (int e = e0 + threadIdx.x; e < e1; e += BLOCK) {
+        int ig = ent_ig[e];
+        C v = plane[ent_pos[e]];
+        C s = ps[ig];
+        double kn = kin[ig];
+        double re = kn * (double)s.x + (double)(v.x * scale);
+        double im = kn * (double)s.y + (double)(v.y * scale);
+        if (accumulate) {
+            C h = hs[ig];
+            re += (double)h.x;
+            im += (double)h.y;
+        }
+        hs[ig] = C{(T)re, (T)im};
+    }
+}
+
+// x pencils of one (j, z-chunk) for NBW consecutive bands: V_loc of the
+// tile stays in registers across the bands.
+template <typename T, int N1, int TZ, int BLOCK, int NBW>
+__global__ void __launch_bounds__(BLOCK)
+k_xpencil_vloc_ct(cx_t<T>* __restrict__ work, int nplanes, int nb, const int* __restrict__ plane_x,
+                  const int* __restrict__ x_to_plane, const int* __restrict__ perm1,
+                  const T* __restrict__ vloc, int n2, int n3, const cx_t<T>* __restrict__ tw_g) {
+    using C = cx_t<T>;
+    constexpr int NE = N1 * TZ;
+    constexpr int PER = (NE + BLOCK - 1) / BLOCK;
+    __shared__ C buf[NE];
+    __shared__ C tw[N1];
+    __shared__ int srcpl[N1];
+    __shared__ int dstpos[N1];
+    const int nzc = (n3 + TZ - 1) / TZ;
+    const int j = blockIdx.x / nzc;
+    const int z0 = (blockIdx.x - j * nzc) * TZ;
+    const int nz = min(TZ, n3 - z0);
+    const int64_t n23 = (int64_t)n2 * n3;
+    if (threadIdx.x < N1) {
+        tw[threadIdx.x] = tw_g[threadIdx.x];
+        srcpl[threadIdx.x] = x_to_plane[threadIdx.x];
+        dstpos[threadIdx.x] = perm1[threadIdx.x] * TZ;
+    }
+    // V_loc(x, j, z0 + z) for the natural-order elements this thread scales
+    T vr[PER];
+    const T* vb = vloc + (int64_t)j * n3 + z0;
+#pragma unroll
+    for (int it = 0; it < PER; ++it) {
+        int idx = threadIdx.x + it * BLOCK;
+        int x = idx / TZ, z = idx - x * TZ;
+        vr[it] = (idx < NE && z < nz) ? vb[(int64_t)x * n23 + z] : T(0);
+    }
+    __syncthreads();
+    for (int bb = 0; bb < NBW; ++bb) {
+        const int b = blockIdx.y * NBW + bb;
+        if (b >= nb) break;
+        cx_t<T>* wb = work + (int64_t)b * nplanes * n23 + (int64_t)j * n3 + z0;
+#pragma unroll
+        for (int it = 0; it < PER; ++it) {
+            int idx = threadIdx.x + it * BLOCK;
+            if (idx < NE) {
+                int x = idx / TZ, z = idx - x * TZ;
+                int pl = srcpl[x];
+                C v = C{0, 0};
+                if (pl >= 0 && z < nz) v = wb[(int64_t)pl * n23 + z];
+                buf[dstpos[x] + z] = v;
+            }
+        }
+        __syncthreads();
+        ct_fft<N1, +1, true, TZ, 1, TZ, BLOCK, T>(buf, tw);     // G2R along x
+#pragma unroll
+        for (int it = 0; it < PER; ++it) {
+            int idx = threadIdx.x + it * BLOCK;
+            if (idx < NE) {
+                C a = buf[idx];
+                buf[idx] = C{a.x * vr[it], a.y * vr[it]};
+            }
+        }
+        __syncthreads();
+        ct_fft<N1, -1, false, TZ, 1, TZ, BLOCK, T>(buf, tw);    // R2G along x
+        for (int idx = threadIdx.x; idx < nplanes * TZ; idx += BLOCK) {
+            int pl = idx / TZ, z = idx - pl * TZ;
+            if (z < nz) wb[(int64_t)pl * n23 + z] = buf[dstpos[plane_x[pl]] + z];
+        }
+        __syncthreads();
+    }
+}
+
+template <int N> struct PencilShape;
+template <> struct PencilShape<90> { static constexpr int TZ = 18, BLOCK = 192; };
+template <> struct PencilShape<48> { static constexpr int TZ = 16, BLOCK = 128; };
+template <> struct PencilShape<36> { static constexpr int TZ = 18, BLOCK = 128; };
+
+constexpr int kBandsPerPencilWG = 4;
+
+static int Plan2N(int n, int which) {
+    switch (n) {
+        case 90: return which == 1 ? Plan2<90>::P1 : Plan2<90>::P2;
+        case 48: return which == 1 ? Plan2<48>::P1 : Plan2<48>::P2;
+        case 36: return which == 1 ? Plan2<36>::P1 : Plan2<36>::P2;
+        default: return -1;
+    }
+}
+
+static bool plan_is(const FftAxis& a, int p1, int p2) {
+    return a.ns == 2 && a.p[0] == p1 && a.p[1] == p2;
+}
+
+template <typename T, int N>
+static hipError_t launch_planes_ct(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld,
+                                   void* hpsi, int64_t ldo, bool accumulate, void* work, bool g2r,
+                                   const cx_t<T>* tw) {
+    using C = cx_t<T>;
+    constexpr int BLOCK = plane_block<N>();
+    const size_t lds = sizeof(C) * ((size_t)N * N + N);
+    dim3 gp(k.nplanes, nb);
+    hipError_t e;
+    if (g2r) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_g2r_ct<T, N, BLOCK>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_plane_g2r_ct<T, N, BLOCK>), gp, dim3(BLOCK), lds, c.stream,
+                           static_cast<const C*>(psi), ld, static_cast<C*>(work), k.nplanes,
+                           static_cast<const int*>(k.plane_ptr.ptr), static_cast<const int*>(k.ent_ig.ptr),
+                           static_cast<const int*>(k.ent_pos.ptr), tw);
+    } else {
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_r2g_ct<T, N, BLOCK>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        T scale = (T)(1.0 / ((double)c.n1 * c.n2 * c.n3));
+        hipLaunchKernelGGL((k_plane_r2g_ct<T, N, BLOCK>), gp, dim3(BLOCK), lds, c.stream,
+                           static_cast<const C*>(work), k.nplanes,
+                           static_cast<const int*>(k.plane_ptr.ptr), static_cast<const int*>(k.ent_ig.ptr),
+                           static_cast<const int*>(k.ent_pos.ptr), static_cast<const C*>(psi), ld,
+                           static_cast<C*>(hpsi), ldo, static_cast<const double*>(k.kin.ptr), scale,
+                           accumulate ? 1 : 0, tw);
+    }
+    return hipGetLastError();
+}
+
+template <typename T, int N>
+static void launch_pencil_ct(Ctx& c, const KptDev& k, int nb, void* work, const T* vl,
+                             const cx_t<T>* tw1) {
+    using C = cx_t<T>;
+    constexpr int TZ = PencilShape<N>::TZ, BLOCK = PencilShape<N>::BLOCK;
+    int nzc = (c.n3 + TZ - 1) / TZ;
+    dim3 gx(c.n2 * nzc, (nb + kBandsPerPencilWG - 1) / kBandsPerPencilWG);
+    hipLaunchKernelGGL((k_xpencil_vloc_ct<T, N, TZ, BLOCK, kBandsPerPencilWG>), gx, dim3(BLOCK), 0,
+                       c.stream, static_cast<C*>(work), k.nplanes, nb,
+                       static_cast<const int*>(k.plane_x.ptr), static_cast<const int*>(k.x_to_plane.ptr),
+                       static_cast<const int*>(c.perm1.ptr), vl, c.n2, c.n3, tw1);
+}
+
 template <typename T>
 static size_t plane_lds_bytes(const Ctx& c) {
     return sizeof(cx_t<T>) * ((size_t)c.n2 * c.n3 + c.n2 + c.n3);
@@ -166,21 +372,55 @@ hipError_t launch_local(Ctx& c, const KptDev& k, int nb, const void* psi, int64_
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pencil);
     if (e != hipSuccess) return e;
 
-    dim3 gp(k.nplanes, nb);
-    timer_begin(c);
-    hipLaunchKernelGGL((k_plane_g2r<T>), gp, dim3(512), lds_plane, c.stream,
+    // compile-time plans for the common square planes / pencil lengths
+    const int np = c.n2;
+    const bool sq = c.n2 == c.n3;
+    auto ct_plane = [&](int n) {
+        return sq && np == n && plan_is(c.ax2, Plan2N(n, 1), Plan2N(n, 2)) &&
+               plan_is(c.ax3, Plan2N(n, 1), Plan2N(n, 2));
+    };
+    auto ct_pencil = [&](int n) { return c.n1 == n && plan_is(c.ax1, Plan2N(n, 1), Plan2N(n, 2)); };
+    const bool pl90 = ct_plane(90), pl48 = ct_plane(48), pl36 = ct_plane(36);
+    const bool pe90 = ct_pencil(90), pe48 = ct_pencil(48), pe36 = ct_pencil(36);
+    if (pl90 || pl48 || pl36) {
+        timer_begin(c);
+        if (pl90) e = launch_planes_ct<T, 90>(c, k, nb, psi, ld, hpsi, ldo, accumulate, work, true, tw2);
+        else if (pl48) e = launch_planes_ct<T, 48>(c, k, nb, psi, ld, hpsi, ldo, accumulate, work, true, tw2);
+        else e = launch_planes_ct<T, 36>(c, k, nb, psi, ld, hpsi, ldo, accumulate, work, true, tw2);
+        timer_end(c, "plane_g2r");
+        if (e != hipSuccess) return e;
+    } else {
+        dim3 gp(k.nplanes, nb);
+        timer_begin(c);
+        hipLaunchKernelGGL((k_plane_g2r<T>), gp, dim3(512), lds_plane, c.stream,
                        static_cast<const C*>(psi), ld, static_cast<C*>(work), k.nplanes,
                        static_cast<const int*>(k.plane_ptr.ptr), static_cast<const int*>(k.ent_ig.ptr),
                        static_cast<const int*>(k.ent_pos.ptr), c.ax2, c.ax3, tw2, tw3);
-    timer_end(c, "plane_g2r");
-    int nzc = (c.n3 + TZ - 1) / TZ;
-    dim3 gx(c.n2 * nzc, nb);
+        timer_end(c, "plane_g2r");
+    }
     timer_begin(c);
-    hipLaunchKernelGGL((k_xpencil_vloc<T, TZ>), gx, dim3(256), lds_pencil, c.stream,
-                       static_cast<C*>(work), k.nplanes, static_cast<const int*>(k.plane_x.ptr),
-                       static_cast<const int*>(k.x_to_plane.ptr), static_cast<const int*>(c.perm1.ptr),
-                       vl, c.ax1, c.n2, c.n3, tw1);
+    if (pe90) launch_pencil_ct<T, 90>(c, k, nb, work, vl, tw1);
+    else if (pe48) launch_pencil_ct<T, 48>(c, k, nb, work, vl, tw1);
+    else if (pe36) launch_pencil_ct<T, 36>(c, k, nb, work, vl, tw1);
+    else {
+        int nzc = (c.n3 + TZ - 1) / TZ;
+        dim3 gx(c.n2 * nzc, nb);
+        hipLaunchKernelGGL((k_xpencil_vloc<T, TZ>), gx, dim3(256), lds_pencil, c.stream,
+                           static_cast<C*>(work), k.nplanes, static_cast<const int*>(k.plane_x.ptr),
+                           static_cast<const int*>(k.x_to_plane.ptr), static_cast<const int*>(c.perm1.ptr),
+                           vl, c.ax1, c.n2, c.n3, tw1);
+    }
     timer_end(c, "xpencil_vloc");
+    if (pl90 || pl48 || pl36) {
+        timer_begin(c);
+        if (pl90) e = launch_planes_ct<T, 90>(c, k, nb, psi, ld, hpsi, ldo, accumulate, work, false, tw2);
+        else if (pl48) e = launch_planes_ct<T, 48>(c, k, nb, psi, ld, hpsi, ldo, accumulate, work, false, tw2);
+        else e = launch_planes_ct<T, 36>(c, k, nb, psi, ld, hpsi, ldo, accumulate, work, false, tw2);
+        timer_end(c, "plane_r2g");
+        if (e != hipSuccess) return e;
+        return hipGetLastError();
+    }
+    dim3 gp(k.nplanes, nb);
     T scale = (T)(1.0 / ((double)c.n1 * c.n2 * c.n3));
     timer_begin(c);
     hipLaunchKernelGGL((k_plane_r2g<T>), gp, dim3(512), lds_plane, c.stream,

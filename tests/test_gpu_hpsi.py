@@ -262,3 +262,15 @@ def test_multi_k_ragged():
         ref = oracle.hpsi(wl.cell, wl.kpts[ik], wl.psi[ik], nthreads=8)
         assert band_rel(outs[ik].cpu().numpy(), ref) <= 1e-12
     H.close()
+
+
+@pytest.mark.parametrize("grid", [(28, 30, 32), (42, 45, 40), (24, 20, 36), (48, 49, 50)])
+def test_fft_radix_plans(grid):
+    """Grids whose axes use radices 2..12 (incl. 7) in the LDS plan."""
+    wl = synth.custom(*synth.fcc_primitive(10.26), grid, 12.0,
+                      [np.array([0.13, 0.0, -0.27])], 4, seed=sum(grid))
+    got = run_h(wl)
+    ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0])
+    e = band_rel(got, ref)
+    log(f"grid {grid} rel err {e:.2e}")
+    assert e <= 1e-12
