@@ -210,79 +210,112 @@ k_plane_r2g_ct(const cx_t<T>* __restrict__ work, int nplanes, const int* __restr
     }
 }
 
-// x pencils of one (j, z-chunk) for NBW consecutive bands: V_loc of the
-// tile stays in registers across the bands.
-template <typename T, int N1, int TZ, int BLOCK, int NBW>
-__global__ void __launch_bounds__(BLOCK)
-k_xpencil_vloc_ct(cx_t<T>* __restrict__ work, int nplanes, int nb, const int* __restrict__ plane_x,
-                  const int* __restrict__ x_to_plane, const int* __restrict__ perm1,
+// x pencils of one (j, z-chunk) for NBW consecutive bands, as register
+// FFTs.  N1 = P1 * P2 (Plan2).  Thread (z, role):
+//   G2R (DIT)  role b < P2 loads x = P2 u + b (u < P1) straight from the
+//              plane workspace, DFT_P1 -> LDS transpose -> role k1 < P1
+//              gathers Z[u'][k1], twiddle W^(u' k1), DFT_P2: it now holds
+//              natural x = k1 + P1 q' (q' < P2);
+//   V          multiply by V_loc(x, j, z) kept in registers across bands;
+//   R2G (DIF)  DFT_P2 on the same values, twiddle W^(q k1) -> LDS transpose
+//              -> role b gathers, DFT_P1 -> holds x = P2 q + b, stored to
+//              the active planes only.
+// Two LDS round trips per band (double-buffered transposes, two barriers);
+// the next band's loads are issued before the current band's arithmetic.
+template <typename T, int N1, int TZ, int NBW>
+__global__ void __launch_bounds__(TZ * cmax(Plan2<N1>::P1, Plan2<N1>::P2))
+k_xpencil_vloc_ct(cx_t<T>* __restrict__ work, int nplanes, int nb, const int* __restrict__ x_to_plane,
                   const T* __restrict__ vloc, int n2, int n3, const cx_t<T>* __restrict__ tw_g) {
     using C = cx_t<T>;
-    constexpr int NE = N1 * TZ;
-    constexpr int PER = (NE + BLOCK - 1) / BLOCK;
-    __shared__ C buf[NE];
+    constexpr int P1 = Plan2<N1>::P1, P2 = Plan2<N1>::P2;
+    __shared__ C ex[2][N1 * TZ];
     __shared__ C tw[N1];
     __shared__ int srcpl[N1];
-    __shared__ int dstpos[N1];
     const int nzc = (n3 + TZ - 1) / TZ;
     const int j = blockIdx.x / nzc;
     const int z0 = (blockIdx.x - j * nzc) * TZ;
-    const int nz = min(TZ, n3 - z0);
+    const int z = threadIdx.x % TZ;
+    const int role = threadIdx.x / TZ;
+    const bool zv = z0 + z < n3;
     const int64_t n23 = (int64_t)n2 * n3;
-    if (threadIdx.x < N1) {
-        tw[threadIdx.x] = tw_g[threadIdx.x];
-        srcpl[threadIdx.x] = x_to_plane[threadIdx.x];
-        dstpos[threadIdx.x] = perm1[threadIdx.x] * TZ;
+    for (int i = threadIdx.x; i < N1; i += TZ * cmax(P1, P2)) {
+        tw[i] = tw_g[i];
+        srcpl[i] = x_to_plane[i];
     }
-    // V_loc(x, j, z0 + z) for the natural-order elements this thread scales
-    T vr[PER];
-    const T* vb = vloc + (int64_t)j * n3 + z0;
+    // V at the natural x = role + P1 q' this thread scales
+    T vr[P2];
 #pragma unroll
-    for (int it = 0; it < PER; ++it) {
-        int idx = threadIdx.x + it * BLOCK;
-        int x = idx / TZ, z = idx - x * TZ;
-        vr[it] = (idx < NE && z < nz) ? vb[(int64_t)x * n23 + z] : T(0);
+    for (int q = 0; q < P2; ++q) {
+        const int x = role + P1 * q;
+        vr[q] = (role < P1 && zv) ? vloc[(int64_t)x * n23 + (int64_t)j * n3 + z0 + z] : T(0);
     }
     __syncthreads();
-    for (int bb = 0; bb < NBW; ++bb) {
-        const int b = blockIdx.y * NBW + bb;
-        if (b >= nb) break;
-        cx_t<T>* wb = work + (int64_t)b * nplanes * n23 + (int64_t)j * n3 + z0;
+    const int64_t col = (int64_t)j * n3 + z0 + z;
+    C y[P1];
+    auto load = [&](int b) {
+        const C* wb = work + (int64_t)b * nplanes * n23 + col;
 #pragma unroll
-        for (int it = 0; it < PER; ++it) {
-            int idx = threadIdx.x + it * BLOCK;
-            if (idx < NE) {
-                int x = idx / TZ, z = idx - x * TZ;
-                int pl = srcpl[x];
-                C v = C{0, 0};
-                if (pl >= 0 && z < nz) v = wb[(int64_t)pl * n23 + z];
-                buf[dstpos[x] + z] = v;
+        for (int u = 0; u < P1; ++u) {
+            const int x = P2 * u + role;
+            const int pl = (role < P2) ? srcpl[x] : -1;
+            y[u] = (pl >= 0 && zv) ? wb[(int64_t)pl * n23] : C{0, 0};
+        }
+    };
+    const int bfirst = blockIdx.y * NBW;
+    const int bend = min(nb, bfirst + NBW);
+    if (bfirst < bend) load(bfirst);
+    for (int b = bfirst; b < bend; ++b) {
+        C v[P1];
+#pragma unroll
+        for (int u = 0; u < P1; ++u) v[u] = y[u];
+        if (b + 1 < bend) load(b + 1);                      // prefetch the next band
+        // ---- G2R stage 1: DFT_P1 over u, positions b*P1 + q
+        if (role < P2) {
+            dft<P1, +1, T>(v);
+#pragma unroll
+            for (int q = 0; q < P1; ++q) ex[0][(role * P1 + q) * TZ + z] = v[q];
+        }
+        __syncthreads();
+        C w[P2];
+        if (role < P1) {
+            // ---- G2R stage 2: twiddle W^(u' k1) (backward), DFT_P2
+#pragma unroll
+            for (int u = 0; u < P2; ++u) {
+                C a = ex[0][(u * P1 + role) * TZ + z];
+                w[u] = (u * role) ? cmul(a, twid<+1>(tw, u * role)) : a;
+            }
+            dft<P2, +1, T>(w);
+            // ---- V_loc(r) at natural x = role + P1 q
+#pragma unroll
+            for (int q = 0; q < P2; ++q) w[q] = C{w[q].x * vr[q], w[q].y * vr[q]};
+            // ---- R2G stage 1 (DIF): DFT_P2, twiddle W^(q k1) (forward)
+            dft<P2, -1, T>(w);
+#pragma unroll
+            for (int q = 0; q < P2; ++q) {
+                C a = (q * role) ? cmul(w[q], twid<-1>(tw, q * role)) : w[q];
+                ex[1][(q * P1 + role) * TZ + z] = a;
             }
         }
         __syncthreads();
-        ct_fft<N1, +1, true, TZ, 1, TZ, BLOCK, T>(buf, tw);     // G2R along x
+        if (role < P2 && zv) {
+            // ---- R2G stage 2: DFT_P1, natural x = P2 q + b
 #pragma unroll
-        for (int it = 0; it < PER; ++it) {
-            int idx = threadIdx.x + it * BLOCK;
-            if (idx < NE) {
-                C a = buf[idx];
-                buf[idx] = C{a.x * vr[it], a.y * vr[it]};
+            for (int u = 0; u < P1; ++u) v[u] = ex[1][(role * P1 + u) * TZ + z];
+            dft<P1, -1, T>(v);
+            C* wb = work + (int64_t)b * nplanes * n23 + col;
+#pragma unroll
+            for (int q = 0; q < P1; ++q) {
+                const int pl = srcpl[P2 * q + role];
+                if (pl >= 0) wb[(int64_t)pl * n23] = v[q];
             }
         }
-        __syncthreads();
-        ct_fft<N1, -1, false, TZ, 1, TZ, BLOCK, T>(buf, tw);    // R2G along x
-        for (int idx = threadIdx.x; idx < nplanes * TZ; idx += BLOCK) {
-            int pl = idx / TZ, z = idx - pl * TZ;
-            if (z < nz) wb[(int64_t)pl * n23 + z] = buf[dstpos[plane_x[pl]] + z];
-        }
-        __syncthreads();
     }
 }
 
 template <int N> struct PencilShape;
-template <> struct PencilShape<90> { static constexpr int TZ = 18, BLOCK = 192; };
-template <> struct PencilShape<48> { static constexpr int TZ = 16, BLOCK = 128; };
-template <> struct PencilShape<36> { static constexpr int TZ = 18, BLOCK = 128; };
+template <> struct PencilShape<90> { static constexpr int TZ = 18; };
+template <> struct PencilShape<48> { static constexpr int TZ = 16; };
+template <> struct PencilShape<36> { static constexpr int TZ = 18; };
 
 constexpr int kBandsPerPencilWG = 4;
 
@@ -335,13 +368,13 @@ template <typename T, int N>
 static void launch_pencil_ct(Ctx& c, const KptDev& k, int nb, void* work, const T* vl,
                              const cx_t<T>* tw1) {
     using C = cx_t<T>;
-    constexpr int TZ = PencilShape<N>::TZ, BLOCK = PencilShape<N>::BLOCK;
+    constexpr int TZ = PencilShape<N>::TZ;
+    constexpr int BLOCK = TZ * cmax(Plan2<N>::P1, Plan2<N>::P2);
     int nzc = (c.n3 + TZ - 1) / TZ;
     dim3 gx(c.n2 * nzc, (nb + kBandsPerPencilWG - 1) / kBandsPerPencilWG);
-    hipLaunchKernelGGL((k_xpencil_vloc_ct<T, N, TZ, BLOCK, kBandsPerPencilWG>), gx, dim3(BLOCK), 0,
-                       c.stream, static_cast<C*>(work), k.nplanes, nb,
-                       static_cast<const int*>(k.plane_x.ptr), static_cast<const int*>(k.x_to_plane.ptr),
-                       static_cast<const int*>(c.perm1.ptr), vl, c.n2, c.n3, tw1);
+    hipLaunchKernelGGL((k_xpencil_vloc_ct<T, N, TZ, kBandsPerPencilWG>), gx, dim3(BLOCK), 0, c.stream,
+                       static_cast<C*>(work), k.nplanes, nb, static_cast<const int*>(k.x_to_plane.ptr),
+                       vl, c.n2, c.n3, tw1);
 }
 
 template <typename T>

@@ -251,41 +251,65 @@ k_gemm_proj(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restric
     V accR[2][2], accI[2][2];
     for (int r = 0; r < 2; ++r)
         for (int q = 0; q < 2; ++q) { accR[r][q] = V{0, 0, 0, 0}; accI[r][q] = V{0, 0, 0, 0}; }
+    // Rows / columns past M / N are clamped onto the last one: they only feed
+    // outputs that are never stored, so the loads need no branch.
     const C* ar[2];
     const C* br[2];
-    bool av[2], bv[2];
     for (int r = 0; r < 2; ++r) {
-        int row = m0 + r * 16 + i;
-        av[r] = row < M;
-        ar[r] = A + (int64_t)(av[r] ? row : 0) * lda;
-        int col = n0 + r * 16 + i;
-        bv[r] = col < N;
-        br[r] = B + (int64_t)(bv[r] ? col : 0) * ldb;
+        ar[r] = A + (int64_t)min(m0 + r * 16 + i, M - 1) * lda;
+        br[r] = B + (int64_t)min(n0 + r * 16 + i, N - 1) * ldb;
     }
-    for (int kb = ks; kb < ke; kb += 16) {
-        C a[2][4], b[2][4];
+    // register double buffering: the next 16-wide k slab is loaded while
+    // the MFMAs of the current one run
+    C a[2][2][4], b[2][2][4];
+    // k past the split end reads a clamped address and is zeroed by a select
+    auto load = [&](int buf, int kb) {
         const int kl = kb + kk * 4;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const bool kv = (kl + s) < ke;
+            const int kc = kv ? kl + s : ke - 1;
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
-                a[r][s] = (av[r] && kv) ? ar[r][kl + s] : C{0, 0};
-                b[r][s] = (bv[r] && kv) ? br[r][kl + s] : C{0, 0};
+                const C x = ar[r][kc];
+                const C y = br[r][kc];
+                a[buf][r][s] = kv ? x : C{0, 0};
+                b[buf][r][s] = y;
             }
         }
+    };
+    // the four real products of a complex MAC go to two accumulators; each
+    // term sweeps all four tiles so dependent MFMAs are 8 apart
+    auto mma = [&](int buf) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+        for (int s = 0; s < 4; ++s) {
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    accR[r][q] = mfma16(a[r][s].x, b[q][s].x, accR[r][q]);
-                    accR[r][q] = mfma16(a[r][s].y, b[q][s].y, accR[r][q]);
-                    accI[r][q] = mfma16(a[r][s].x, b[q][s].y, accI[r][q]);
-                    accI[r][q] = mfma16(-a[r][s].y, b[q][s].x, accI[r][q]);
-                }
+                for (int q = 0; q < 2; ++q) accR[r][q] = mfma16(a[buf][r][s].x, b[buf][q][s].x, accR[r][q]);
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) accI[r][q] = mfma16(a[buf][r][s].x, b[buf][q][s].y, accI[r][q]);
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) accR[r][q] = mfma16(a[buf][r][s].y, b[buf][q][s].y, accR[r][q]);
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) accI[r][q] = mfma16(-a[buf][r][s].y, b[buf][q][s].x, accI[r][q]);
+        }
+    };
+    if (ks < ke) load(0, ks);
+    int kb = ks;
+    for (; kb + 16 < ke; kb += 32) {
+        load(1, kb + 16);
+        mma(0);
+        if (kb + 32 < ke) load(0, kb + 32);
+        mma(1);
     }
+    if (kb < ke) mma(0);
     C* po = part + (int64_t)blockIdx.z * Mtot * N;
     for (int r = 0; r < 2; ++r)
         for (int q = 0; q < 2; ++q)
@@ -334,30 +358,59 @@ k_gemm_apply(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restri
     V accR[2][2], accI[2][2];
     for (int r = 0; r < 2; ++r)
         for (int q = 0; q < 2; ++q) { accR[r][q] = V{0, 0, 0, 0}; accI[r][q] = V{0, 0, 0, 0}; }
-    bool mv[2], nv[2];
+    // clamped columns of A / B (see k_gemm_proj)
+    int mc[2], nc[2];
     for (int r = 0; r < 2; ++r) {
-        mv[r] = (m0 + r * 16 + i) < M;
-        nv[r] = (n0 + r * 16 + i) < N;
+        mc[r] = min(m0 + r * 16 + i, M - 1);
+        nc[r] = min(n0 + r * 16 + i, N - 1);
     }
-    for (int k0 = 0; k0 < K; k0 += 4) {
-        const int k = k0 + kk;
-        const bool kv = k < K;
-        C a[2], b[2];
+    // four k-steps of 4 per slab, double buffered in registers
+    C a[2][4][2], b[2][4][2];
+    auto load = [&](int buf, int kb) {
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            a[r] = (kv && mv[r]) ? A[(int64_t)k * lda + m0 + r * 16 + i] : C{0, 0};
-            b[r] = (kv && nv[r]) ? B[(int64_t)k * ldb + n0 + r * 16 + i] : C{0, 0};
-        }
+        for (int s = 0; s < 4; ++s) {
+            const int k = kb + 4 * s + kk;
+            const bool kv = k < K;
+            const int kc = kv ? k : K - 1;
 #pragma unroll
-        for (int r = 0; r < 2; ++r)
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                accR[r][q] = mfma16(a[r].x, b[q].x, accR[r][q]);
-                accR[r][q] = mfma16(-a[r].y, b[q].y, accR[r][q]);
-                accI[r][q] = mfma16(a[r].x, b[q].y, accI[r][q]);
-                accI[r][q] = mfma16(a[r].y, b[q].x, accI[r][q]);
+            for (int r = 0; r < 2; ++r) {
+                const C x = A[(int64_t)kc * lda + mc[r]];
+                const C y = B[(int64_t)kc * ldb + nc[r]];
+                a[buf][s][r] = kv ? x : C{0, 0};
+                b[buf][s][r] = y;
             }
+        }
+    };
+    auto mma = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) accR[r][q] = mfma16(a[buf][s][r].x, b[buf][s][q].x, accR[r][q]);
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) accI[r][q] = mfma16(a[buf][s][r].x, b[buf][s][q].y, accI[r][q]);
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) accR[r][q] = mfma16(-a[buf][s][r].y, b[buf][s][q].y, accR[r][q]);
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) accI[r][q] = mfma16(a[buf][s][r].y, b[buf][s][q].x, accI[r][q]);
+        }
+    };
+    if (K > 0) load(0, 0);
+    int kb = 0;
+    for (; kb + 16 < K; kb += 32) {
+        load(1, kb + 16);
+        mma(0);
+        if (kb + 32 < K) load(0, kb + 32);
+        mma(1);
     }
+    if (kb < K) mma(0);
     for (int r = 0; r < 2; ++r)
         for (int q = 0; q < 2; ++q) {
             const int n = n0 + q * 16 + i;
