@@ -1,0 +1,114 @@
+"""k.p effective-mass tensor on top of the device H|psi> and dH/dk|psi>.
+
+For a band n at k (Hartree atomic units, hbar = m_e = 1):
+
+  (1/m*)_ab = d^2 E_n / dk_a dk_b
+            = <n| d^2H/dk_a dk_b |n>
+              + 2 Re sum_{m != n} <n|dH/dk_a|m><m|dH/dk_b|n> / (E_n - E_m)
+
+d^2H/dk_a dk_b is delta_ab (kinetic) plus the second k-derivative of the
+nonlocal term.  The reference declares that Hessian but leaves it empty
+(src/H_psi/hamiltonian_hessian.cpp:48-50); here <n|d^2H|n> is the central
+difference in k of the analytic <n|dH/dk_a(k +- delta e_b)|n>
+(src/H_psi/hamiltonian_gradient.cpp:116-294 on the device), which is
+exact for the kinetic part and O(delta^2) for the nonlocal part.
+
+Velocity matrices, the sum over states and the (E_n - E_m) denominators are
+FP64 in both precisions; only the H|psi> / dH/dk applications follow `prec`.
+States m within `degen_tol` of E_n are left out of the sum (their
+contribution is undefined for a degenerate level; the result is then the
+mass of the level projected out of its degenerate partners).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import synth
+from .hamiltonian import GradHPsi, Hamiltonian
+
+
+@dataclass
+class MassResult:
+    bands: list
+    inv_mass: np.ndarray        # [len(bands), 3, 3]  (1/m*)_ab
+    d2h: np.ndarray             # [len(bands), 3, 3]  <n|d^2H/dk_a dk_b|n>
+    E: np.ndarray               # [nb] eigenvalues used
+    excluded: list              # per band: indices left out as degenerate
+
+    def masses(self):
+        """Principal effective masses (inverse eigenvalues of 1/m*)."""
+        return np.array([1.0 / np.linalg.eigvalsh(0.5 * (m + m.T)) for m in self.inv_mass])
+
+
+def shifted_kpoint(kp: synth.KPoint, A: np.ndarray, dk_cart) -> synth.KPoint:
+    """Same G set at k + dk (Cartesian): K + dk, |K|^2/2, k_frac + dk.A^T/2pi."""
+    dk = np.asarray(dk_cart, dtype=np.float64)
+    K = kp.K + dk[None, :]
+    return synth.KPoint(kfrac=kp.kfrac + dk @ A.T / (2.0 * np.pi), K=np.ascontiguousarray(K),
+                        kin=0.5 * np.einsum("ij,ij->i", K, K), gidx=kp.gidx)
+
+
+def kp_sum(E, V, d2h, n, degen_tol=1e-6):
+    """(1/m*)_ab for band n from eigenvalues E [nb], velocity matrices
+    V[a][m, l] = <m|dH/dk_a|l> (3 x nb x nb) and <n|d^2H|n> (3x3)."""
+    E = np.asarray(E, dtype=np.float64)
+    d = E[n] - E
+    keep = np.abs(d) > degen_tol
+    keep[n] = False
+    M = np.array(d2h, dtype=np.float64).copy()
+    for a in range(3):
+        for b in range(3):
+            t = V[a][n, keep] * V[b][keep, n] / d[keep]
+            M[a, b] += 2.0 * np.real(t.sum())
+    return 0.5 * (M + M.T), np.nonzero(~keep)[0].tolist()
+
+
+def velocity_matrices(H: Hamiltonian, ikpt: int, X: torch.Tensor, prec="fp64"):
+    """V[a] = <m|dH/dk_a|n> (FP64, nb x nb) for the rows of X."""
+    Xp = X.to(torch.complex128 if prec == "fp64" else torch.complex64).contiguous()
+    D = [torch.zeros_like(Xp) for _ in range(3)]
+    H.gradient().at_k(ikpt)(Xp, GradHPsi(*D), prec=prec, overwrite=True)
+    X64 = X.to(torch.complex128)
+    return [(X64.conj() @ d.to(torch.complex128).T).cpu().numpy() for d in D]
+
+
+def d2h_expect(H: Hamiltonian, kp: synth.KPoint, A, X: torch.Tensor, bands, prec="fp64",
+               delta=1e-3, slot=None):
+    """<n| d^2H / dk_a dk_b |n> for the bands (rows of X) by central
+    differences of the analytic gradient; uses one scratch k-point slot
+    (default: one past the highest loaded)."""
+    if slot is None:
+        slot = max(H._npw.keys(), default=-1) + 1
+    Xb = X[bands].to(torch.complex128 if prec == "fp64" else torch.complex64).contiguous()
+    X64 = X[bands].to(torch.complex128)
+    out = np.zeros((len(bands), 3, 3))
+    for b in range(3):
+        vals = []
+        for sgn in (+1.0, -1.0):
+            dk = np.zeros(3)
+            dk[b] = sgn * delta
+            H.load_kpoint(slot, shifted_kpoint(kp, A, dk))
+            D = [torch.zeros_like(Xb) for _ in range(3)]
+            H.gradient().at_k(slot)(Xb, GradHPsi(*D), prec=prec, overwrite=True)
+            vals.append(np.stack([np.real((X64.conj() * d.to(torch.complex128)).sum(1)).cpu().numpy()
+                                  for d in D], axis=1))      # [nbands, a]
+        out[:, :, b] = (vals[0] - vals[1]) / (2.0 * delta)
+    H.release_kpoint(slot)
+    return 0.5 * (out + out.transpose(0, 2, 1))
+
+
+def effective_mass(H: Hamiltonian, ikpt: int, kp: synth.KPoint, A, X: torch.Tensor, E, bands,
+                   prec="fp64", delta=1e-3, degen_tol=1e-6) -> MassResult:
+    """(1/m*) for `bands` from the eigenpairs (X rows, E) of H at ikpt."""
+    E = np.asarray(E.detach().cpu().numpy() if torch.is_tensor(E) else E, dtype=np.float64)
+    V = velocity_matrices(H, ikpt, X, prec)
+    d2 = d2h_expect(H, kp, A, X, bands, prec, delta)
+    inv, excl = [], []
+    for i, n in enumerate(bands):
+        m, ex = kp_sum(E, V, d2[i], n, degen_tol)
+        inv.append(m)
+        excl.append(ex)
+    return MassResult(bands=list(bands), inv_mass=np.array(inv), d2h=d2, E=E, excluded=excl)
