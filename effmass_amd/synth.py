@@ -346,7 +346,7 @@ def build(config: int, nb: int | None = None, nk: int | None = None, seed: int |
     return wl
 
 
-def custom(A, tau, n, ecut_ry, kfracs, nb, seed, species_idx=None, species_list=None):
+def custom(A, tau, n, ecut_ry, kfracs, nb, seed, species_idx=None, species_list=None, with_psi=True):
     """A workload for an arbitrary cell (used by the serving loop)."""
     rng = SplitMix64(seed)
     if species_list is None:
@@ -358,5 +358,6 @@ def custom(A, tau, n, ecut_ry, kfracs, nb, seed, species_idx=None, species_list=
                 species_list=species_list)
     kpts = [g_sphere(cell.A, n, k, ecut_ry) for k in kfracs]
     wl = Workload(name="custom", cell=cell, kpts=kpts, nb=nb, ecut_ry=ecut_ry, seed=seed)
-    wl.psi = [make_psi(kp, nb, rng) for kp in kpts]
+    if with_psi:
+        wl.psi = [make_psi(kp, nb, rng) for kp in kpts]
     return wl

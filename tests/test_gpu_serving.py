@@ -1,0 +1,46 @@
+"""Serving loop on the device: continuous batching over concurrent
+streams, HBM-footprint admission, and results that do not depend on the
+mix (each query alone and among others gives the same masses)."""
+import numpy as np
+import pytest
+
+from effmass_amd import serving
+
+pytestmark = pytest.mark.gpu
+
+
+def _queries():
+    Q = serving.MassQuery
+    return [Q(qid=0, natom=2, ecut_ry=30.0, nb=16, kfracs=[np.array([0.1, 0.2, -0.3])], seed=5),
+            Q(qid=1, natom=8, ecut_ry=30.0, nb=24, kfracs=[np.array([0.0, 0.1, 0.2]), np.array([0.3, 0.0, 0.0])],
+              seed=6),
+            Q(qid=2, natom=2, ecut_ry=40.0, nb=20, kfracs=[np.array([-0.2, 0.1, 0.05])], seed=7),
+            Q(qid=3, natom=8, ecut_ry=40.0, nb=32, kfracs=[np.array([0.15, -0.1, 0.3])], seed=8)]
+
+
+def test_same_result_alone_and_in_the_mix():
+    qs = _queries()
+    alone = {q.qid: serving.run_query(q) for q in (qs[0], qs[3])}
+    ans, stats = serving.serve(qs, slots=4, realtime=False)
+    print(stats, flush=True)
+    assert stats["errors"] == 0
+    assert stats["max_concurrency"] >= 2
+    for qid, a in alone.items():
+        mix = ans[qid]
+        for x, y in zip(a.inv_mass, mix.inv_mass):
+            assert np.abs(np.asarray(x) - np.asarray(y)).max() <= 1e-10 * np.abs(np.asarray(x)).max()
+        assert np.allclose(a.energies, mix.energies, rtol=0, atol=1e-12)
+
+
+def test_admission_by_hbm_footprint():
+    qs = _queries()
+    need = max(serving.estimate_bytes(q) for q in qs)
+    ans, stats = serving.serve(qs, slots=4, realtime=False, hbm_budget=need)
+    print(stats, flush=True)
+    assert stats["errors"] == 0
+    assert stats["peak_admitted_bytes"] <= need      # never more admitted than the budget
+    assert stats["max_concurrency"] < 4              # the budget held some queries back
+    big = serving.MassQuery(qid=9, natom=64, ecut_ry=50.0, nb=256, kfracs=[np.zeros(3)], seed=1)
+    ans2, stats2 = serving.serve([qs[0], big], slots=2, realtime=False, hbm_budget=need)
+    assert ans2[9].error and "budget" in ans2[9].error
+    assert ans2[0].error is None

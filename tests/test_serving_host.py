@@ -1,0 +1,39 @@
+"""Host-side parts of the serving loop: the config-4 query stream, cells
+with 135 bohr^3 per atom, FFT grids, footprint estimates."""
+import numpy as np
+
+from effmass_amd import serving
+
+
+def test_cells_and_grids():
+    for n in serving.SIZES:
+        A, tau = serving.query_cell(n)
+        assert len(tau) == n
+        assert abs(abs(np.linalg.det(A)) / n - 135.0) < 1e-9
+        for ecut in serving.ECUTS:
+            g = serving.grid_for(A, ecut)
+            for m, a in zip(g, A):
+                assert m >= 2 * np.sqrt(ecut) * np.linalg.norm(a) / (2 * np.pi) + 1
+                k = m
+                for p in (2, 3, 5):
+                    while k % p == 0:
+                        k //= p
+                assert k == 1
+
+
+def test_stream_is_seeded_poisson():
+    a = serving.make_stream(200, rate=50.0, seed=5)
+    b = serving.make_stream(200, rate=50.0, seed=5)
+    assert [q.natom for q in a] == [q.natom for q in b]
+    t = np.array([q.arrival for q in a])
+    assert np.all(np.diff(t) > 0)
+    assert 2.5 < t[-1] < 6.0                       # ~200 / 50 s
+    assert {q.natom for q in a} == set(serving.SIZES)
+    assert all(16 <= q.nb <= 256 and 1 <= len(q.kfracs) <= 4 for q in a)
+
+
+def test_footprint_grows_with_size():
+    small = serving.MassQuery(qid=0, natom=2, ecut_ry=30.0, nb=16, kfracs=[np.zeros(3)], seed=1)
+    big = serving.MassQuery(qid=1, natom=16, ecut_ry=50.0, nb=128, kfracs=[np.zeros(3)], seed=1)
+    es, eb = serving.estimate_bytes(small), serving.estimate_bytes(big)
+    assert 0 < es < eb
