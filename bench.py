@@ -51,7 +51,25 @@ def main():
     ap.add_argument("--cpu-bands", type=int, default=32, help="bands in the 1-core CPU sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--timing-steps", type=int, default=3)
+    ap.add_argument("--serve-queries", type=int, default=24,
+                    help="config-4 effective-mass queries replayed at 50/s (0 = skip)")
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
+                    help="time one precision only (profiling runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="run a plain device copy of MB megabytes 5 times and exit "
+                         "(calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 8 * n, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the H|psi> path shards over "
                                    "k-points and bands without collectives (see DESIGN.md), "
@@ -81,8 +99,9 @@ def main():
         for _ in range(steps):
             op(p, h, prec=prec, overwrite=True)
 
+    precs = (args.only,) if args.only else ("fp64", "mixed")
     res = {}
-    for prec in ("fp64", "mixed"):
+    for prec in precs:
         run(prec, args.warmup)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -91,6 +110,9 @@ def main():
         dt = time.perf_counter() - t0
         res[prec] = {"ms_per_step": dt * 1e3 / args.steps, "bands_per_s": nb * args.steps / dt}
 
+    if args.only:
+        print(json.dumps({"only": args.only, **res[args.only]}), flush=True)
+        return
     # energies: FP64 Rayleigh quotients of both paths
     op(psi64, h64, prec="fp64", overwrite=True)
     op(psi32, h32, prec="mixed", overwrite=True)
@@ -113,7 +135,10 @@ def main():
                 continue
             avg_s = ms / cnt * 1e-3
             kind = "fp64" if prec == "fp64" else "fp32"
-            bound, amount = kernel_model(name, "fp64" if prec == "fp64" else "fp32", **geo)
+            # bands per launch: the FFT passes run in band chunks when the
+            # workspace is capped, the projector kernels on all bands
+            g = dict(geo, nb=nb * args.timing_steps // cnt)
+            bound, amount = kernel_model(name, "fp64" if prec == "fp64" else "fp32", **g)
             ent = {"kernel": name, "prec": prec, "avg_us": avg_s * 1e6, "launches": cnt,
                    "share_of_step": ms / cnt / res[prec]["ms_per_step"], "bound": bound}
             if amount is not None:
@@ -160,6 +185,20 @@ def main():
         parity = {"fp64_max_rel_err_vs_oracle": float(rel.max()),
                   "mixed_max_rel_err_vs_oracle": float(rel32.max()), "bands_checked": nsb}
 
+    # serving: Poisson(50/s) effective-mass queries over 2-64 atom cells
+    serving_stats = None
+    if args.serve_queries > 0:
+        from effmass_amd import serving
+        H.close()
+        torch.cuda.empty_cache()
+        qs = serving.make_stream(args.serve_queries, rate=50.0, seed=2024)
+        serving.run_query(serving.make_stream(1, seed=7)[0])       # warm-up (kernel loading)
+        _, serving_stats = serving.serve(qs, slots=args.serve_slots, prec="mixed")
+        serving_stats["rate_per_s"] = 50.0
+        serving_stats["workload"] = ("Poisson 50/s; cells of 2/8/16/32/64 atoms at 135 bohr^3/atom, "
+                                     "ecut 30/40/50 Ry, 16-256 bands, 1-4 k-points; m* of band nb/2 "
+                                     "at every k (LOBPCG + k.p, mixed precision)")
+
     out = {
         "metric": "Hψ bands/s, 64-atom cell, mixed vs FP64 at 1e-6 Ry; p50/p99 m* query latency",
         "value": res["mixed"]["bands_per_s"],
@@ -184,6 +223,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "serving": serving_stats,
         "setup_s": setup_s,
     }
     print(json.dumps(out), flush=True)

@@ -248,9 +248,17 @@ k_gemm_proj(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restric
     const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
     const int ks = blockIdx.z * kchunk;
     const int ke = min(K, ks + kchunk);
-    V accR[2][2], accI[2][2];
+    // Gauss's three-product complex MAC: for conj(P) psi with a = Re P,
+    // b = -Im P, c = Re psi, d = Im psi:
+    //   k1 += c (a + b),  k2 += a (d - c),  k3 += b (c + d)
+    //   Re = k1 - k3,     Im = k1 + k2
+    V k1[2][2], k2[2][2], k3[2][2];
     for (int r = 0; r < 2; ++r)
-        for (int q = 0; q < 2; ++q) { accR[r][q] = V{0, 0, 0, 0}; accI[r][q] = V{0, 0, 0, 0}; }
+        for (int q = 0; q < 2; ++q) {
+            k1[r][q] = V{0, 0, 0, 0};
+            k2[r][q] = V{0, 0, 0, 0};
+            k3[r][q] = V{0, 0, 0, 0};
+        }
     // Rows / columns past M / N are clamped onto the last one: they only feed
     // outputs that are never stored, so the loads need no branch.
     const C* ar[2];
@@ -278,27 +286,28 @@ k_gemm_proj(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restric
             }
         }
     };
-    // the four real products of a complex MAC go to two accumulators; each
-    // term sweeps all four tiles so dependent MFMAs are 8 apart
     auto mma = [&](int buf) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
+            T sa[2], sc[2], dc[2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                sa[r] = a[buf][r][s].x - a[buf][r][s].y;             // a + b
+                sc[r] = b[buf][r][s].x + b[buf][r][s].y;             // c + d
+                dc[r] = b[buf][r][s].y - b[buf][r][s].x;             // d - c
+            }
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) accR[r][q] = mfma16(a[buf][r][s].x, b[buf][q][s].x, accR[r][q]);
+                for (int q = 0; q < 2; ++q) k1[r][q] = mfma16(sa[r], b[buf][q][s].x, k1[r][q]);
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) accI[r][q] = mfma16(a[buf][r][s].x, b[buf][q][s].y, accI[r][q]);
+                for (int q = 0; q < 2; ++q) k2[r][q] = mfma16(a[buf][r][s].x, dc[q], k2[r][q]);
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) accR[r][q] = mfma16(a[buf][r][s].y, b[buf][q][s].y, accR[r][q]);
-#pragma unroll
-            for (int r = 0; r < 2; ++r)
-#pragma unroll
-                for (int q = 0; q < 2; ++q) accI[r][q] = mfma16(-a[buf][r][s].y, b[buf][q][s].x, accI[r][q]);
+                for (int q = 0; q < 2; ++q) k3[r][q] = mfma16(-a[buf][r][s].y, sc[q], k3[r][q]);
         }
     };
     if (ks < ke) load(0, ks);
@@ -317,7 +326,8 @@ k_gemm_proj(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restric
                 int row = m0 + r * 16 + out_row<T>(kk, v);
                 int col = n0 + q * 16 + i;
                 if (row < M && col < N)
-                    po[(int64_t)(row0 + row) * N + col] = C{accR[r][q][v], accI[r][q][v]};
+                    po[(int64_t)(row0 + row) * N + col] =
+                        C{k1[r][q][v] - k3[r][q][v], k1[r][q][v] + k2[r][q][v]};
             }
 }
 
@@ -355,9 +365,15 @@ k_gemm_apply(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restri
     const int i = lane & 15, kk = lane >> 4;
     const int m0 = blockIdx.x * 64 + (wave & 1) * 32;
     const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
-    V accR[2][2], accI[2][2];
+    // Gauss's three-product complex MAC with a = Re P, b = Im P, c = Re C,
+    // d = Im C: k1 += c (a + b), k2 += a (d - c), k3 += b (c + d).
+    V k1[2][2], k2[2][2], k3[2][2];
     for (int r = 0; r < 2; ++r)
-        for (int q = 0; q < 2; ++q) { accR[r][q] = V{0, 0, 0, 0}; accI[r][q] = V{0, 0, 0, 0}; }
+        for (int q = 0; q < 2; ++q) {
+            k1[r][q] = V{0, 0, 0, 0};
+            k2[r][q] = V{0, 0, 0, 0};
+            k3[r][q] = V{0, 0, 0, 0};
+        }
     // clamped columns of A / B (see k_gemm_proj)
     int mc[2], nc[2];
     for (int r = 0; r < 2; ++r) {
@@ -384,22 +400,25 @@ k_gemm_apply(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restri
     auto mma = [&](int buf) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
+            T sa[2], sc[2], dc[2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                sa[r] = a[buf][s][r].x + a[buf][s][r].y;
+                sc[r] = b[buf][s][r].x + b[buf][s][r].y;
+                dc[r] = b[buf][s][r].y - b[buf][s][r].x;
+            }
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) accR[r][q] = mfma16(a[buf][s][r].x, b[buf][s][q].x, accR[r][q]);
+                for (int q = 0; q < 2; ++q) k1[r][q] = mfma16(sa[r], b[buf][s][q].x, k1[r][q]);
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) accI[r][q] = mfma16(a[buf][s][r].x, b[buf][s][q].y, accI[r][q]);
+                for (int q = 0; q < 2; ++q) k2[r][q] = mfma16(a[buf][s][r].x, dc[q], k2[r][q]);
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) accR[r][q] = mfma16(-a[buf][s][r].y, b[buf][s][q].y, accR[r][q]);
-#pragma unroll
-            for (int r = 0; r < 2; ++r)
-#pragma unroll
-                for (int q = 0; q < 2; ++q) accI[r][q] = mfma16(a[buf][s][r].y, b[buf][s][q].x, accI[r][q]);
+                for (int q = 0; q < 2; ++q) k3[r][q] = mfma16(a[buf][s][r].y, sc[q], k3[r][q]);
         }
     };
     if (K > 0) load(0, 0);
@@ -418,7 +437,8 @@ k_gemm_apply(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restri
             for (int v = 0; v < 4; ++v) {
                 const int m = m0 + r * 16 + out_row<T>(kk, v);
                 if (m >= M) continue;
-                double re = (double)accR[r][q][v], im = (double)accI[r][q][v];
+                double re = (double)(k1[r][q][v] - k3[r][q][v]);
+                double im = (double)(k1[r][q][v] + k2[r][q][v]);
                 if (diag) {
                     C s = src[(int64_t)n * lds + m];
                     re += diag[m] * (double)s.x;
