@@ -209,7 +209,7 @@ static int check_kpt(Ctx& c, int ikpt) {
 // Bytes of FFT plane workspace per band.
 template <typename T>
 static size_t work_per_band(const Ctx& c, const KptDev& k) {
-    return (size_t)k.nplanes * c.n2 * c.n3 * sizeof(cx_t<T>);
+    return (size_t)k.nplanes * ws_plane<T>(c.n2, c.n3) * sizeof(cx_t<T>);
 }
 
 static constexpr size_t kWorkCap = size_t(3) << 30;   // FFT workspace cap (bytes)
@@ -296,6 +296,9 @@ int effm_ctx_create(effm_ctx** out, int device, uint64_t hbm_budget) {
     auto* h = new (std::nothrow) effm_ctx;
     if (!h) return kErrNoMem;
     h->c.device = device;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+        h->c.ncu = ncu;
     h->c.budget = hbm_budget;
     h->c.stream = nullptr;   // legacy default stream until effm_set_stream
     *out = h;

@@ -81,6 +81,7 @@ struct Timer {
 
 struct Ctx {
     int device = 0;
+    int ncu = 256;                      // compute units of the device
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
     uint64_t used = 0;
@@ -110,6 +111,9 @@ struct Ctx {
     int nq_max = 0;
 
     std::vector<KptDev> kpts;
+
+    // per kernel: resident workgroups per CU (dynamic LDS attribute set once)
+    std::map<const void*, int> occupancy;
 
     // scratch (grown on demand)
     DevBuf work;       // FFT plane workspace
@@ -146,6 +150,22 @@ struct TableDev {
     double norm;     // 1/sqrt(Omega)
     double dbeta0;   // normalised dbeta/dq at q = 0
 };
+
+// Row pitch (complex elements) of the FFT plane workspace.  FP32 rows are
+// padded to a 128-byte line (the x-pencil pass then reads whole lines);
+// FP64 rows stay unpadded (measured faster on the 90^3 grid: the plane
+// passes are limited by their own LDS work, not by line alignment).
+template <typename T>
+constexpr int ws_pitch(int n3) {
+    constexpr int q = 128 / (int)(2 * sizeof(T));
+    return sizeof(T) == 8 ? n3 : (n3 + q - 1) / q * q;
+}
+
+// Plane stride (complex elements) of the workspace.
+template <typename T>
+constexpr int64_t ws_plane(int n2, int n3) {
+    return (int64_t)n2 * ws_pitch<T>(n3);
+}
 
 // ---- launchers (k_local.hip) --------------------------------------------
 template <typename T>

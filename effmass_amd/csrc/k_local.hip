@@ -18,6 +18,8 @@
 //                    hpsi = [hpsi +] kin*psi + V psi / N.
 // Planes outside the sphere's x-range are never written or read: this is
 // the zero-padding the reference's dense 3D FFT spends on them.
+#include <algorithm>
+
 #include "effm_internal.hpp"
 #include "fft_ct.hpp"
 
@@ -28,7 +30,7 @@ __global__ void __launch_bounds__(512)
 k_plane_g2r(const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict__ work, int nplanes,
             const int* __restrict__ plane_ptr, const int* __restrict__ ent_ig,
             const int* __restrict__ ent_pos, FftAxis a2, FftAxis a3,
-            const cx_t<T>* __restrict__ tw2, const cx_t<T>* __restrict__ tw3) {
+            const cx_t<T>* __restrict__ tw2, const cx_t<T>* __restrict__ tw3, int wp) {
     using C = cx_t<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n2 = a2.n, n3 = a3.n, n23 = n2 * n3;
@@ -47,8 +49,11 @@ k_plane_g2r(const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict__ w
     __syncthreads();
     fft_lds<+1, true, T>(plane, a3, 1, n3, n2, t3);    // along k (rows)
     fft_lds<+1, true, T>(plane, a2, n3, 1, n3, t2);    // along j (columns)
-    C* out = work + ((int64_t)b * nplanes + p) * n23;
-    for (int i = threadIdx.x; i < n23; i += blockDim.x) out[i] = plane[i];
+    C* out = work + ((int64_t)b * nplanes + p) * ws_plane<T>(n2, n3);
+    for (int i = threadIdx.x; i < n23; i += blockDim.x) {
+        int j = i / n3;
+        out[j * wp + (i - j * n3)] = plane[i];
+    }
 }
 
 template <typename T, int TZ>
@@ -56,7 +61,7 @@ __global__ void __launch_bounds__(256)
 k_xpencil_vloc(cx_t<T>* __restrict__ work, int nplanes, const int* __restrict__ plane_x,
                const int* __restrict__ x_to_plane, const int* __restrict__ perm1,
                const T* __restrict__ vloc, FftAxis a1, int n2, int n3,
-               const cx_t<T>* __restrict__ tw1) {
+               const cx_t<T>* __restrict__ tw1, int wp) {
     using C = cx_t<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n1 = a1.n;
@@ -68,13 +73,14 @@ k_xpencil_vloc(cx_t<T>* __restrict__ work, int nplanes, const int* __restrict__ 
     const int nz = min(TZ, n3 - z0);
     const int b = blockIdx.y;
     const int64_t n23 = (int64_t)n2 * n3;
-    cx_t<T>* wb = work + (int64_t)b * nplanes * n23 + (int64_t)j * n3 + z0;
+    const int64_t wps = ws_plane<T>(n2, n3);   // workspace plane stride
+    cx_t<T>* wb = work + (int64_t)b * nplanes * wps + (int64_t)j * wp + z0;
     for (int i = threadIdx.x; i < n1; i += blockDim.x) t1[i] = tw1[i];
     for (int idx = threadIdx.x; idx < n1 * TZ; idx += blockDim.x) {
         int x = idx / TZ, z = idx - x * TZ;
         int pl = x_to_plane[x];
         C v = C{0, 0};
-        if (pl >= 0 && z < nz) v = wb[(int64_t)pl * n23 + z];
+        if (pl >= 0 && z < nz) v = wb[(int64_t)pl * wps + z];
         buf[perm1[x] * TZ + z] = v;
     }
     __syncthreads();
@@ -92,7 +98,7 @@ k_xpencil_vloc(cx_t<T>* __restrict__ work, int nplanes, const int* __restrict__ 
     fft_lds<-1, false, T>(buf, a1, TZ, 1, TZ, t1);     // R2G along x
     for (int idx = threadIdx.x; idx < nplanes * TZ; idx += blockDim.x) {
         int pl = idx / TZ, z = idx - pl * TZ;
-        if (z < nz) wb[(int64_t)pl * n23 + z] = buf[perm1[plane_x[pl]] * TZ + z];
+        if (z < nz) wb[(int64_t)pl * wps + z] = buf[perm1[plane_x[pl]] * TZ + z];
     }
 }
 
@@ -102,7 +108,7 @@ k_plane_r2g(const cx_t<T>* __restrict__ work, int nplanes, const int* __restrict
             const int* __restrict__ ent_ig, const int* __restrict__ ent_pos,
             const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict__ hpsi, int64_t ldo,
             const double* __restrict__ kin, T scale, int accumulate, FftAxis a2, FftAxis a3,
-            const cx_t<T>* __restrict__ tw2, const cx_t<T>* __restrict__ tw3) {
+            const cx_t<T>* __restrict__ tw2, const cx_t<T>* __restrict__ tw3, int wp) {
     using C = cx_t<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n2 = a2.n, n3 = a3.n, n23 = n2 * n3;
@@ -111,8 +117,11 @@ k_plane_r2g(const cx_t<T>* __restrict__ work, int nplanes, const int* __restrict
     C* t3 = t2 + n2;
     const int p = blockIdx.x;
     const int b = blockIdx.y;
-    const C* in = work + ((int64_t)b * nplanes + p) * n23;
-    for (int i = threadIdx.x; i < n23; i += blockDim.x) plane[i] = in[i];
+    const C* in = work + ((int64_t)b * nplanes + p) * ws_plane<T>(n2, n3);
+    for (int i = threadIdx.x; i < n23; i += blockDim.x) {
+        int j = i / n3;
+        plane[i] = in[j * wp + (i - j * n3)];
+    }
     for (int i = threadIdx.x; i < n2; i += blockDim.x) t2[i] = tw2[i];
     for (int i = threadIdx.x; i < n3; i += blockDim.x) t3[i] = tw3[i];
     __syncthreads();
@@ -143,70 +152,209 @@ k_plane_r2g(const cx_t<T>* __restrict__ work, int nplanes, const int* __restrict
 //  Specialised kernels for square n2 = n3 = N planes and N1-point x pencils
 //  with compile-time two-stage plans (fft_ct.hpp).  Same data flow as above.
 // ---------------------------------------------------------------------------
+// Persistent plane kernels: each workgroup walks items = (plane, band) with
+// stride gridDim.x.  In FP32 the next item's global loads are issued before
+// this item's global stores, so both drain behind the next item's LDS work;
+// in FP64 the registers that would hold them are taken by the FFT itself.
+template <typename T>
+constexpr bool kPipePlanes = sizeof(T) == 4;
+
 template <typename T, int N, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
 k_plane_g2r_ct(const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict__ work, int nplanes,
-               const int* __restrict__ plane_ptr, const int* __restrict__ ent_ig,
+               int nitems, const int* __restrict__ plane_ptr, const int* __restrict__ ent_ig,
                const int* __restrict__ ent_pos, const cx_t<T>* __restrict__ tw_g) {
     using C = cx_t<T>;
     constexpr int N2 = N * N;
+    constexpr int WP = ws_pitch<T>(N);
+    constexpr int NP = N * WP;                          // rows written (pad columns included)
+    constexpr int64_t PS = ws_plane<T>(N, N);           // workspace plane stride
+    constexpr int NOUT = (NP + BLOCK - 1) / BLOCK;
+    constexpr int PF = 2;                               // prefetched scatter entries per thread
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* plane = reinterpret_cast<C*>(smem_raw);
     C* tw = plane + N2;
-    const int p = blockIdx.x;
-    const int b = blockIdx.y;
-#pragma unroll
-    for (int i = threadIdx.x; i < N2; i += BLOCK) plane[i] = C{0, 0};
     if (threadIdx.x < N) tw[threadIdx.x] = tw_g[threadIdx.x];
-    __syncthreads();
-    const C* ps = psi + (int64_t)b * ld;
-    const int e0 = plane_ptr[p], e1 = plane_ptr[p + 1];
-    for (int e = e0 + threadIdx.x; e < e1; e += BLOCK) plane[ent_pos[e]] = ps[ent_ig[e]];
-    __syncthreads();
-    ct_fft<N, +1, true, 1, N, N, BLOCK, T>(plane, tw);     // along k (rows)
-    ct_fft<N, +1, true, N, 1, N, BLOCK, T>(plane, tw);     // along j (columns)
-    C* out = work + ((int64_t)b * nplanes + p) * N2;
+    // FP64 prefetches nothing (no registers to spare next to the FFT)
+    constexpr int NPF = kPipePlanes<T> ? PF : 0;
+    C pv[PF];
+    int pp[PF];
+    int e0 = 0, e1 = 0;
+    auto prefetch = [&](int it) {
+        const int p = it % nplanes, b = it / nplanes;
+        e0 = plane_ptr[p];
+        e1 = plane_ptr[p + 1];
 #pragma unroll
-    for (int i = threadIdx.x; i < N2; i += BLOCK) out[i] = plane[i];
+        for (int f = 0; f < NPF; ++f) {
+            const int e = e0 + threadIdx.x + f * BLOCK;
+            pp[f] = -1;
+            if (e < e1) {
+                pp[f] = ent_pos[e];
+                pv[f] = psi[(int64_t)b * ld + ent_ig[e]];
+            }
+        }
+    };
+    int item = blockIdx.x;
+    if (item < nitems) prefetch(item);
+    for (; item < nitems; item += gridDim.x) {
+        const int p = item % nplanes, b = item / nplanes;
+        constexpr int NZ = (N2 + BLOCK - 1) / BLOCK;
+#pragma unroll
+        for (int it = 0; it < NZ; ++it) {
+            const int i = threadIdx.x + it * BLOCK;
+            if (NZ * BLOCK == N2 || i < N2) plane[i] = C{0, 0};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < NPF; ++f)
+            if (pp[f] >= 0) plane[pp[f]] = pv[f];
+        for (int e = e0 + threadIdx.x + NPF * BLOCK; e < e1; e += BLOCK)
+            plane[ent_pos[e]] = psi[(int64_t)b * ld + ent_ig[e]];
+        __syncthreads();
+        ct_fft<N, +1, true, 1, N, N, BLOCK, T>(plane, tw);     // along k (rows)
+        ct_fft<N, +1, true, N, 1, N, BLOCK, T>(plane, tw);     // along j (columns)
+        C* o = work + ((int64_t)b * nplanes + p) * PS;
+        const int nxt = item + gridDim.x;
+        if constexpr (kPipePlanes<T>) {
+            // FP32: plane -> registers, then the next item's loads, then the stores
+            C ov[NOUT];
+#pragma unroll
+            for (int it = 0; it < NOUT; ++it) {
+                const int idx = threadIdx.x + it * BLOCK;
+                const int j = idx / WP;
+                const int kk = idx - j * WP;
+                ov[it] = plane[min(j, N - 1) * N + min(kk, N - 1)];   // pad columns carry junk
+            }
+            __syncthreads();
+            if (nxt < nitems) prefetch(nxt);
+#pragma unroll
+            for (int it = 0; it < NOUT; ++it) {
+                const int idx = threadIdx.x + it * BLOCK;
+                if (NOUT * BLOCK == NP || idx < NP) o[idx] = ov[it];
+            }
+        } else {
+            // FP64: a plane of registers does not fit next to the FFT's; store
+            // straight from LDS, then issue the next item's loads
+#pragma unroll
+            for (int it = 0; it < NOUT; ++it) {
+                const int idx = threadIdx.x + it * BLOCK;
+                const int j = idx / WP;
+                const int kk = idx - j * WP;
+                if (NOUT * BLOCK == NP || idx < NP) o[idx] = plane[min(j, N - 1) * N + min(kk, N - 1)];
+            }
+            __syncthreads();
+            if (nxt < nitems) prefetch(nxt);
+        }
+    }
 }
 
 template <typename T, int N, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-k_plane_r2g_ct(const cx_t<T>* __restrict__ work, int nplanes, const int* __restrict__ plane_ptr,
-               const int* __restrict__ ent_ig, const int* __restrict__ ent_pos,
-               const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict__ hpsi, int64_t ldo,
-               const double* __restrict__ kin, T scale, int accumulate,
-               const cx_t<T>* __restrict__ tw_g) {
+k_plane_r2g_ct(const cx_t<T>* __restrict__ work, int nplanes, int nitems,
+               const int* __restrict__ plane_ptr, const int* __restrict__ ent_ig,
+               const int* __restrict__ ent_pos, const cx_t<T>* __restrict__ psi, int64_t ld,
+               cx_t<T>* __restrict__ hpsi, int64_t ldo, const double* __restrict__ kin, T scale,
+               int accumulate, const cx_t<T>* __restrict__ tw_g) {
     using C = cx_t<T>;
     constexpr int N2 = N * N;
+    constexpr int WP = ws_pitch<T>(N);
+    constexpr int NP = N * WP;
+    constexpr int64_t PS = ws_plane<T>(N, N);
+    constexpr int NIN = (NP + BLOCK - 1) / BLOCK;
+    constexpr int PF = 2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* plane = reinterpret_cast<C*>(smem_raw);
     C* tw = plane + N2;
-    const int p = blockIdx.x;
-    const int b = blockIdx.y;
-    const C* in = work + ((int64_t)b * nplanes + p) * N2;
-#pragma unroll
-    for (int i = threadIdx.x; i < N2; i += BLOCK) plane[i] = in[i];
     if (threadIdx.x < N) tw[threadIdx.x] = tw_g[threadIdx.x];
-    __syncthreads();
-    ct_fft<N, -1, false, N, 1, N, BLOCK, T>(plane, tw);    // along j
-    ct_fft<N, -1, false, 1, N, N, BLOCK, T>(plane, tw);    // along k
-    const C* ps = psi + (int64_t)b * ld;
-    C* hs = hpsi + (int64_t)b * ldo;
-    const int e0 = plane_ptr[p], e1 = plane_ptr[p + 1];
-    for (int e = e0 + threadIdx.x; e < e1; e += BLOCK) {
-        int ig = ent_ig[e];
-        C v = plane[ent_pos[e]];
-        C s = ps[ig];
-        double kn = kin[ig];
-        double re = kn * (double)s.x + (double)(v.x * scale);
-        double im = kn * (double)s.y + (double)(v.y * scale);
+    auto plane_in = [&](int it) {
+        const int p = it % nplanes, b = it / nplanes;
+        return work + ((int64_t)b * nplanes + p) * PS;
+    };
+    // FP32: the next plane is prefetched into registers behind this one's
+    // FFT; FP64 loads each plane at the top of its item
+    constexpr int NIV = kPipePlanes<T> ? NIN : 1;
+    C iv[NIV];
+    auto prefetch = [&](int it) {
+        const C* in = plane_in(it);
+#pragma unroll
+        for (int q = 0; q < NIV; ++q) {
+            const int idx = threadIdx.x + q * BLOCK;
+            iv[q] = in[min(idx, NP - 1)];
+        }
+    };
+    auto to_lds = [&](int q, const C& v) {
+        const int idx = threadIdx.x + q * BLOCK;
+        const int j = idx / WP;
+        const int kk = idx - j * WP;
+        if (kk < N && j < N) plane[j * N + kk] = v;
+    };
+    auto gather = [&](int e, const C* ps, C* hs) {
+        const int ig = ent_ig[e];
+        const C v = plane[ent_pos[e]];
+        const C sv = ps[ig];
+        double re = kin[ig] * (double)sv.x + (double)(v.x * scale);
+        double im = kin[ig] * (double)sv.y + (double)(v.y * scale);
         if (accumulate) {
-            C h = hs[ig];
+            const C h = hs[ig];
             re += (double)h.x;
             im += (double)h.y;
         }
         hs[ig] = C{(T)re, (T)im};
+    };
+    int item = blockIdx.x;
+    if constexpr (kPipePlanes<T>)
+        if (item < nitems) prefetch(item);
+    for (; item < nitems; item += gridDim.x) {
+        const int p = item % nplanes, b = item / nplanes;
+        if constexpr (kPipePlanes<T>) {
+#pragma unroll
+            for (int q = 0; q < NIN; ++q) to_lds(q, iv[q]);
+        } else {
+            const C* in = plane_in(item);
+#pragma unroll
+            for (int q = 0; q < NIN; ++q) to_lds(q, in[min((int)threadIdx.x + q * BLOCK, NP - 1)]);
+        }
+        __syncthreads();
+        const int e0 = plane_ptr[p], e1 = plane_ptr[p + 1];
+        const C* ps = psi + (int64_t)b * ld;
+        C* hs = hpsi + (int64_t)b * ldo;
+        if constexpr (kPipePlanes<T>) {
+            // this item's gather inputs, then the next plane, both in flight
+            // during the FFT
+            int gi[PF], gp[PF];
+            C gs[PF], gh[PF];
+            double gk[PF];
+#pragma unroll
+            for (int f = 0; f < PF; ++f) {
+                const int e = e0 + threadIdx.x + f * BLOCK;
+                gi[f] = -1;
+                if (e < e1) {
+                    gi[f] = ent_ig[e];
+                    gp[f] = ent_pos[e];
+                    gs[f] = ps[gi[f]];
+                    gk[f] = kin[gi[f]];
+                    gh[f] = accumulate ? hs[gi[f]] : C{0, 0};
+                }
+            }
+            const int nxt = item + gridDim.x;
+            if (nxt < nitems) prefetch(nxt);
+            ct_fft<N, -1, false, N, 1, N, BLOCK, T>(plane, tw);    // along j
+            ct_fft<N, -1, false, 1, N, N, BLOCK, T>(plane, tw);    // along k
+#pragma unroll
+            for (int f = 0; f < PF; ++f) {
+                if (gi[f] < 0) continue;
+                const C v = plane[gp[f]];
+                double re = gk[f] * (double)gs[f].x + (double)(v.x * scale) + (double)gh[f].x;
+                double im = gk[f] * (double)gs[f].y + (double)(v.y * scale) + (double)gh[f].y;
+                hs[gi[f]] = C{(T)re, (T)im};
+            }
+            for (int e = e0 + threadIdx.x + PF * BLOCK; e < e1; e += BLOCK) gather(e, ps, hs);
+        } else {
+            ct_fft<N, -1, false, N, 1, N, BLOCK, T>(plane, tw);    // along j
+            ct_fft<N, -1, false, 1, N, N, BLOCK, T>(plane, tw);    // along k
+            for (int e = e0 + threadIdx.x; e < e1; e += BLOCK) gather(e, ps, hs);
+        }
+        __syncthreads();
     }
 }
 
@@ -225,7 +373,7 @@ k_plane_r2g_ct(const cx_t<T>* __restrict__ work, int nplanes, const int* __restr
 template <typename T, int N1, int TZ, int NBW>
 __global__ void __launch_bounds__(TZ * cmax(Plan2<N1>::P1, Plan2<N1>::P2))
 k_xpencil_vloc_ct(cx_t<T>* __restrict__ work, int nplanes, int nb, const int* __restrict__ x_to_plane,
-                  const T* __restrict__ vloc, int n2, int n3, const cx_t<T>* __restrict__ tw_g) {
+                  const T* __restrict__ vloc, int n2, int n3, const cx_t<T>* __restrict__ tw_g, int wp) {
     using C = cx_t<T>;
     constexpr int P1 = Plan2<N1>::P1, P2 = Plan2<N1>::P2;
     __shared__ C ex[2][N1 * TZ];
@@ -250,15 +398,16 @@ k_xpencil_vloc_ct(cx_t<T>* __restrict__ work, int nplanes, int nb, const int* __
         vr[q] = (role < P1 && zv) ? vloc[(int64_t)x * n23 + (int64_t)j * n3 + z0 + z] : T(0);
     }
     __syncthreads();
-    const int64_t col = (int64_t)j * n3 + z0 + z;
+    const int64_t wps = ws_plane<T>(n2, n3);           // workspace plane stride
+    const int64_t col = (int64_t)j * wp + z0 + z;
     C y[P1];
     auto load = [&](int b) {
-        const C* wb = work + (int64_t)b * nplanes * n23 + col;
+        const C* wb = work + (int64_t)b * nplanes * wps + col;
 #pragma unroll
         for (int u = 0; u < P1; ++u) {
             const int x = P2 * u + role;
             const int pl = (role < P2) ? srcpl[x] : -1;
-            y[u] = (pl >= 0 && zv) ? wb[(int64_t)pl * n23] : C{0, 0};
+            y[u] = (pl >= 0 && zv) ? wb[(int64_t)pl * wps] : C{0, 0};
         }
     };
     const int bfirst = blockIdx.y * NBW;
@@ -302,20 +451,20 @@ k_xpencil_vloc_ct(cx_t<T>* __restrict__ work, int nplanes, int nb, const int* __
 #pragma unroll
             for (int u = 0; u < P1; ++u) v[u] = ex[1][(role * P1 + u) * TZ + z];
             dft<P1, -1, T>(v);
-            C* wb = work + (int64_t)b * nplanes * n23 + col;
+            C* wb = work + (int64_t)b * nplanes * wps + col;
 #pragma unroll
             for (int q = 0; q < P1; ++q) {
                 const int pl = srcpl[P2 * q + role];
-                if (pl >= 0) wb[(int64_t)pl * n23] = v[q];
+                if (pl >= 0) wb[(int64_t)pl * wps] = v[q];
             }
         }
     }
 }
 
-template <int N> struct PencilShape;
-template <> struct PencilShape<90> { static constexpr int TZ = 18; };
-template <> struct PencilShape<48> { static constexpr int TZ = 16; };
-template <> struct PencilShape<36> { static constexpr int TZ = 18; };
+// z-chunk width of the pencil kernel (measured: FP32 prefers chunks of one
+// 128-byte line, FP64 on the 90-point grid five chunks of 18)
+template <typename T, int N> struct PencilShape { static constexpr int TZ = 16; };
+template <> struct PencilShape<double, 90> { static constexpr int TZ = 18; };
 
 constexpr int kBandsPerPencilWG = 4;
 
@@ -332,6 +481,24 @@ static bool plan_is(const FftAxis& a, int p1, int p2) {
     return a.ns == 2 && a.p[0] == p1 && a.p[1] == p2;
 }
 
+// Persistent grid: as many workgroups as the occupancy calculator says are
+// resident at once (LDS, VGPRs, wave slots), capped by the (plane, band)
+// items.  The dynamic-LDS attribute and the occupancy are set up once per
+// kernel and context.
+static hipError_t persistent_grid(Ctx& c, const void* fn, size_t lds, int block, int nitems, int* grid) {
+    auto it = c.occupancy.find(fn);
+    if (it == c.occupancy.end()) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        it = c.occupancy.emplace(fn, per_cu).first;
+    }
+    *grid = std::max(1, std::min(nitems, c.ncu * it->second));
+    return hipSuccess;
+}
+
 template <typename T, int N>
 static hipError_t launch_planes_ct(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld,
                                    void* hpsi, int64_t ldo, bool accumulate, void* work, bool g2r,
@@ -339,23 +506,25 @@ static hipError_t launch_planes_ct(Ctx& c, const KptDev& k, int nb, const void* 
     using C = cx_t<T>;
     constexpr int BLOCK = plane_block<N>();
     const size_t lds = sizeof(C) * ((size_t)N * N + N);
-    dim3 gp(k.nplanes, nb);
+    const int nitems = k.nplanes * nb;
     hipError_t e;
     if (g2r) {
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_g2r_ct<T, N, BLOCK>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int grid = 0;
+        e = persistent_grid(c, reinterpret_cast<const void*>(&k_plane_g2r_ct<T, N, BLOCK>), lds, BLOCK,
+                            nitems, &grid);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_plane_g2r_ct<T, N, BLOCK>), gp, dim3(BLOCK), lds, c.stream,
-                           static_cast<const C*>(psi), ld, static_cast<C*>(work), k.nplanes,
+        hipLaunchKernelGGL((k_plane_g2r_ct<T, N, BLOCK>), dim3(grid), dim3(BLOCK), lds, c.stream,
+                           static_cast<const C*>(psi), ld, static_cast<C*>(work), k.nplanes, nitems,
                            static_cast<const int*>(k.plane_ptr.ptr), static_cast<const int*>(k.ent_ig.ptr),
                            static_cast<const int*>(k.ent_pos.ptr), tw);
     } else {
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_r2g_ct<T, N, BLOCK>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int grid = 0;
+        e = persistent_grid(c, reinterpret_cast<const void*>(&k_plane_r2g_ct<T, N, BLOCK>), lds, BLOCK,
+                            nitems, &grid);
         if (e != hipSuccess) return e;
         T scale = (T)(1.0 / ((double)c.n1 * c.n2 * c.n3));
-        hipLaunchKernelGGL((k_plane_r2g_ct<T, N, BLOCK>), gp, dim3(BLOCK), lds, c.stream,
-                           static_cast<const C*>(work), k.nplanes,
+        hipLaunchKernelGGL((k_plane_r2g_ct<T, N, BLOCK>), dim3(grid), dim3(BLOCK), lds, c.stream,
+                           static_cast<const C*>(work), k.nplanes, nitems,
                            static_cast<const int*>(k.plane_ptr.ptr), static_cast<const int*>(k.ent_ig.ptr),
                            static_cast<const int*>(k.ent_pos.ptr), static_cast<const C*>(psi), ld,
                            static_cast<C*>(hpsi), ldo, static_cast<const double*>(k.kin.ptr), scale,
@@ -368,13 +537,13 @@ template <typename T, int N>
 static void launch_pencil_ct(Ctx& c, const KptDev& k, int nb, void* work, const T* vl,
                              const cx_t<T>* tw1) {
     using C = cx_t<T>;
-    constexpr int TZ = PencilShape<N>::TZ;
+    constexpr int TZ = PencilShape<T, N>::TZ;
     constexpr int BLOCK = TZ * cmax(Plan2<N>::P1, Plan2<N>::P2);
     int nzc = (c.n3 + TZ - 1) / TZ;
     dim3 gx(c.n2 * nzc, (nb + kBandsPerPencilWG - 1) / kBandsPerPencilWG);
     hipLaunchKernelGGL((k_xpencil_vloc_ct<T, N, TZ, kBandsPerPencilWG>), gx, dim3(BLOCK), 0, c.stream,
                        static_cast<C*>(work), k.nplanes, nb, static_cast<const int*>(k.x_to_plane.ptr),
-                       vl, c.n2, c.n3, tw1);
+                       vl, c.n2, c.n3, tw1, ws_pitch<T>(c.n3));
 }
 
 template <typename T>
@@ -428,7 +597,7 @@ hipError_t launch_local(Ctx& c, const KptDev& k, int nb, const void* psi, int64_
         hipLaunchKernelGGL((k_plane_g2r<T>), gp, dim3(512), lds_plane, c.stream,
                        static_cast<const C*>(psi), ld, static_cast<C*>(work), k.nplanes,
                        static_cast<const int*>(k.plane_ptr.ptr), static_cast<const int*>(k.ent_ig.ptr),
-                       static_cast<const int*>(k.ent_pos.ptr), c.ax2, c.ax3, tw2, tw3);
+                       static_cast<const int*>(k.ent_pos.ptr), c.ax2, c.ax3, tw2, tw3, ws_pitch<T>(c.n3));
         timer_end(c, "plane_g2r");
     }
     timer_begin(c);
@@ -441,7 +610,7 @@ hipError_t launch_local(Ctx& c, const KptDev& k, int nb, const void* psi, int64_
         hipLaunchKernelGGL((k_xpencil_vloc<T, TZ>), gx, dim3(256), lds_pencil, c.stream,
                            static_cast<C*>(work), k.nplanes, static_cast<const int*>(k.plane_x.ptr),
                            static_cast<const int*>(k.x_to_plane.ptr), static_cast<const int*>(c.perm1.ptr),
-                           vl, c.ax1, c.n2, c.n3, tw1);
+                           vl, c.ax1, c.n2, c.n3, tw1, ws_pitch<T>(c.n3));
     }
     timer_end(c, "xpencil_vloc");
     if (pl90 || pl48 || pl36) {
@@ -461,7 +630,7 @@ hipError_t launch_local(Ctx& c, const KptDev& k, int nb, const void* psi, int64_
                        static_cast<const int*>(k.plane_ptr.ptr), static_cast<const int*>(k.ent_ig.ptr),
                        static_cast<const int*>(k.ent_pos.ptr), static_cast<const C*>(psi), ld,
                        static_cast<C*>(hpsi), ldo, static_cast<const double*>(k.kin.ptr), scale,
-                       accumulate ? 1 : 0, c.ax2, c.ax3, tw2, tw3);
+                       accumulate ? 1 : 0, c.ax2, c.ax3, tw2, tw3, ws_pitch<T>(c.n3));
     timer_end(c, "plane_r2g");
     return hipGetLastError();
 }

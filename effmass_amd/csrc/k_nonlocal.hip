@@ -228,12 +228,25 @@ hipError_t launch_build_proj(Ctx& c, KptDev& k, bool grad) {
 template hipError_t launch_build_proj<double>(Ctx&, KptDev&, bool);
 template hipError_t launch_build_proj<float>(Ctx&, KptDev&, bool);
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on
+// XCD i % 8), and each XCD has its own L2.  xcd_order(i, G) renumbers a 1D
+// grid so that consecutive logical ids run on one XCD; tiles that read the
+// same operand rows are given consecutive ids and so share one L2.
+constexpr int kXcds = 8;
+__device__ inline int xcd_order(int i, int G) {
+    const int per = G / kXcds;
+    if (i >= per * kXcds) return i;
+    return (i % kXcds) * per + i / kXcds;
+}
+
 // ---------------------------------------------------------------------------
 //  GEMM 1: part[split][row0 + m][n] = sum_{k in split} conj(A[m][k]) B[n][k]
 //  A: [M][lda] (projectors, G contiguous), B: [N][ldb] (bands, G contiguous).
 //  Workgroup = 4 waves, 64 x 64 tile; wave = 2 x 2 blocks of 16 x 16.
 //  Each lane owns 4 consecutive k of its row per 16-wide k step, so both
-//  operands are read as contiguous 4-element runs.
+//  operands are read as contiguous 4-element runs.  1D grid of
+//  tiles x splits; the tiles of one split are consecutive (one XCD), so its
+//  slabs of A and B are fetched into that XCD's L2 once.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -244,9 +257,12 @@ k_gemm_proj(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restric
     using V = typename Acc<T>::type;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, kk = lane >> 4;
-    const int m0 = blockIdx.x * 64 + (wave & 1) * 32;
-    const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
-    const int ks = blockIdx.z * kchunk;
+    const int nx = (M + 63) / 64, nt = nx * ((N + 63) / 64);
+    const int L = xcd_order(blockIdx.x, gridDim.x);
+    const int split = L / nt, t = L - split * nt;
+    const int m0 = (t % nx) * 64 + (wave & 1) * 32;
+    const int n0 = (t / nx) * 64 + (wave >> 1) * 32;
+    const int ks = split * kchunk;
     const int ke = min(K, ks + kchunk);
     // Gauss's three-product complex MAC: for conj(P) psi with a = Re P,
     // b = -Im P, c = Re psi, d = Im psi:
@@ -319,7 +335,7 @@ k_gemm_proj(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restric
         mma(1);
     }
     if (kb < ke) mma(0);
-    C* po = part + (int64_t)blockIdx.z * Mtot * N;
+    C* po = part + (int64_t)split * Mtot * N;
     for (int r = 0; r < 2; ++r)
         for (int q = 0; q < 2; ++q)
             for (int v = 0; v < 4; ++v) {
@@ -352,6 +368,8 @@ __global__ void k_reduce_scale(const cx_t<T>* __restrict__ part, int nsplit, int
 // ---------------------------------------------------------------------------
 //  GEMM 2: out[n][m] (+)= diag[m] * src[n][m] + sum_k A[k][m] B[k][n]
 //  A: [K][lda] (projectors, G contiguous), B: [K][ldb] (scaled <p|psi>).
+//  1D grid; the column tiles of one row tile are consecutive (one XCD), so
+//  each projector slab is fetched into one L2.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -363,8 +381,10 @@ k_gemm_apply(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restri
     using V = typename Acc<T>::type;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, kk = lane >> 4;
-    const int m0 = blockIdx.x * 64 + (wave & 1) * 32;
-    const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
+    const int ny = (N + 63) / 64;
+    const int L = xcd_order(blockIdx.x, gridDim.x);
+    const int m0 = (L / ny) * 64 + (wave & 1) * 32;
+    const int n0 = (L % ny) * 64 + (wave >> 1) * 32;
     // Gauss's three-product complex MAC with a = Re P, b = Im P, c = Re C,
     // d = Im C: k1 += c (a + b), k2 += a (d - c), k3 += b (c + d).
     V k1[2][2], k2[2][2], k3[2][2];
@@ -474,7 +494,7 @@ static hipError_t gemm_proj(Ctx& c, const cx_t<T>* A, int M, int npw, const cx_t
                             int64_t ld, int nb, cx_t<T>* part, int Mtot, int row0, int nsplit) {
     int kchunk = (npw + nsplit - 1) / nsplit;
     kchunk = (kchunk + 15) / 16 * 16;
-    dim3 grid((M + 63) / 64, (nb + 63) / 64, nsplit);
+    dim3 grid(((M + 63) / 64) * ((nb + 63) / 64) * nsplit);
     hipLaunchKernelGGL((k_gemm_proj<T>), grid, dim3(256), 0, c.stream, A, (int64_t)npw, psi, ld,
                        M, nb, npw, kchunk, part, Mtot, row0);
     return hipGetLastError();
@@ -484,7 +504,7 @@ template <typename T>
 static hipError_t gemm_apply(Ctx& c, const cx_t<T>* A, int npw, const cx_t<T>* B, int nb,
                              int K, cx_t<T>* out, int64_t ldo, bool acc, const double* diag,
                              const cx_t<T>* src, int64_t lds) {
-    dim3 grid((npw + 63) / 64, (nb + 63) / 64);
+    dim3 grid(((npw + 63) / 64) * ((nb + 63) / 64));
     hipLaunchKernelGGL((k_gemm_apply<T>), grid, dim3(256), 0, c.stream, A, (int64_t)npw, B,
                        (int64_t)nb, npw, nb, K, out, ldo, acc ? 1 : 0, diag, src, lds);
     return hipGetLastError();
