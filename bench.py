@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
                     help="time one precision only (profiling runs)")
+    ap.add_argument("--config2-steps", type=int, default=3,
+                    help="timed steps of the 32-k-point batched H|psi> + dH/dk leg (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="run a plain device copy of MB megabytes 5 times and exit "
                          "(calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -185,11 +187,14 @@ def main():
         parity = {"fp64_max_rel_err_vs_oracle": float(rel.max()),
                   "mixed_max_rel_err_vs_oracle": float(rel32.max()), "bands_checked": nsb}
 
+    H.close()
+    # 32 k-points of the 64-atom cell in batched launches, graph replays
+    config2 = config2_leg(args.config2_steps) if args.config2_steps > 0 else None
+
     # serving: Poisson(50/s) effective-mass queries over 2-64 atom cells
     serving_stats = None
     if args.serve_queries > 0:
         from effmass_amd import serving
-        H.close()
         torch.cuda.empty_cache()
         qs = serving.make_stream(args.serve_queries, rate=50.0, seed=2024)
         serving.run_query(serving.make_stream(1, seed=7)[0])       # warm-up (kernel loading)
@@ -223,11 +228,68 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "config2": config2,
         "serving": serving_stats,
         "setup_s": setup_s,
     }
     print(json.dumps(out), flush=True)
+
+
+def config2_leg(steps, warmup=1):
+    """H|psi> and dH/dk_{x,y,z}|psi> for 32 k-points (4x4x2 grid, ragged npw
+    36,816-37,073) x 128 bands of the 64-atom cell: effm_hpsi_multi and
+    effm_dhdk_multi (up to 16 k-points per launch), replayed as captured
+    hipGraphs; psi is seeded device noise (throughput does not depend on
+    the values).  Also times the same work as one call per k-point."""
+    import torch
+    from effmass_amd import synth
+    from effmass_amd.hamiltonian import GradHPsi, Hamiltonian
+    wl = synth.build(2, with_psi=False)
+    H = Hamiltonian(wl.cell, device=0)
+    for ik, kp in enumerate(wl.kpts):
+        H.load_kpoint(ik, kp)
+    ks = list(range(len(wl.kpts)))
+    nb = wl.nb
+    gen = torch.Generator(device="cuda:0").manual_seed(2024)
+    res = {"workload": "Si64 cell, 4x4x2 k-grid (32 k, npw %d-%d), 128 bands per k; step = H|psi> + "
+                       "dH/dk_x,y,z|psi> on every k" % (min(k.npw for k in wl.kpts), max(k.npw for k in wl.kpts)),
+           "kpoints": len(ks), "bands_per_k": nb}
+    for prec, dt in (("mixed", torch.complex64), ("fp64", torch.complex128)):
+        psis = [torch.randn(nb, kp.npw, dtype=dt, device="cuda:0", generator=gen) for kp in wl.kpts]
+        hs = [torch.empty_like(p) for p in psis]
+        gr = GradHPsi([torch.empty_like(p) for p in psis], [torch.empty_like(p) for p in psis],
+                      [torch.empty_like(p) for p in psis])
+
+        def step(graph=True):
+            H.apply_multi(ks, psis, hs, prec=prec, overwrite=True, graph=graph)
+            H.gradient_multi(ks, psis, gr, prec=prec, overwrite=True, graph=graph)
+
+        def per_k():
+            for ik in ks:
+                H.at_k(ik)(psis[ik], hs[ik], prec=prec, overwrite=True)
+                H.gradient().at_k(ik)(psis[ik], GradHPsi(gr.x[ik], gr.y[ik], gr.z[ik]), prec=prec,
+                                      overwrite=True)
+
+        def timed(fn, n):
+            for _ in range(warmup):
+                fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(n):
+                fn()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / n
+
+        t_graph = timed(step, steps)
+        t_eager = timed(lambda: step(False), steps)
+        t_loop = timed(per_k, steps)
+        applies = len(ks) * nb
+        res[prec] = {"ms_per_step": t_graph * 1e3, "k_band_steps_per_s": applies / t_graph,
+                     "eager_batched_ms": t_eager * 1e3, "per_k_calls_ms": t_loop * 1e3}
+        del psis, hs, gr
+        torch.cuda.empty_cache()
     H.close()
+    return res
 
 
 def _cpu_model():

@@ -45,6 +45,7 @@ static void dfree(Ctx& c, DevBuf& b) {
     if (b.ptr) {
         (void)hipFree(b.ptr);
         c.used -= b.bytes;
+        ++c.epoch;
     }
     b.ptr = nullptr;
     b.bytes = 0;
@@ -64,6 +65,7 @@ static int dalloc(Ctx& c, DevBuf& b, size_t bytes, const char* what) {
     }
     b.bytes = bytes;
     c.used += bytes;
+    ++c.epoch;
     return kOk;
 }
 
@@ -139,7 +141,7 @@ static void twiddles(int n, std::vector<double2>& w64, std::vector<float2>& w32)
 }
 
 void timer_begin(Ctx& c) {
-    if (!c.timing) return;
+    if (!c.timing || c.capturing) return;
     hipEvent_t e;
     if (!c.event_pool.empty()) {
         e = c.event_pool.back();
@@ -153,7 +155,7 @@ void timer_begin(Ctx& c) {
 }
 
 void timer_end(Ctx& c, const char* name) {
-    if (!c.timing || !c.cur_start) return;
+    if (!c.timing || c.capturing || !c.cur_start) return;
     hipEvent_t e;
     if (!c.event_pool.empty()) {
         e = c.event_pool.back();
@@ -212,7 +214,7 @@ static size_t work_per_band(const Ctx& c, const KptDev& k) {
     return (size_t)k.nplanes * ws_plane<T>(c.n2, c.n3) * sizeof(cx_t<T>);
 }
 
-static constexpr size_t kWorkCap = size_t(3) << 30;   // FFT workspace cap (bytes)
+static constexpr size_t kWorkCap = size_t(8) << 30;   // FFT workspace cap (bytes)
 
 template <typename T>
 static int ensure_proj(Ctx& c, KptDev& k, bool grad) {
@@ -233,48 +235,160 @@ static int ensure_proj(Ctx& c, KptDev& k, bool grad) {
     return kOk;
 }
 
-template <typename T>
-static int run_hpsi(Ctx& c, KptDev& k, int nb, const void* psi, int64_t ld, void* hpsi,
-                    int64_t ldo, bool acc) {
-    using C = cx_t<T>;
-    int rc = ensure_proj<T>(c, k, false);
-    if (rc) return rc;
-    const size_t wpb = work_per_band<T>(c, k);
+// Workspace cap: kWorkCap (EFFM_WORK_MB overrides), and within the HBM
+// budget next to everything else the context holds.
+static size_t work_cap(const Ctx& c) {
     size_t cap = kWorkCap;
     if (const char* e = std::getenv("EFFM_WORK_MB")) cap = std::max<size_t>(1, std::strtoull(e, nullptr, 10)) << 20;
-    int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)nb, cap / std::max<size_t>(wpb, 1)));
-    if ((rc = ensure(c, c.work, wpb * chunk, "FFT workspace"))) return rc;
-    for (int b0 = 0; b0 < nb; b0 += chunk) {
-        int nbc = std::min(chunk, nb - b0);
-        HIP_TRY(launch_local<T>(c, k, nbc, static_cast<const C*>(psi) + (int64_t)b0 * ld, ld,
-                                static_cast<C*>(hpsi) + (int64_t)b0 * ldo, ldo, acc, c.work.ptr),
-                "local potential kernels");
+    if (c.budget) {
+        const size_t others = c.used - c.work.bytes;
+        cap = std::min(cap, c.budget > others ? (size_t)(c.budget - others) : (size_t)0);
     }
-    if (c.nch() > 0) {
-        size_t pb = vnl_partial_bytes(c.nch(), nb, k.npw, sizeof(C));
-        if ((rc = ensure(c, c.partial, pb, "projection partial sums"))) return rc;
-        if ((rc = ensure(c, c.cbuf, (size_t)c.nch() * nb * sizeof(C), "projections"))) return rc;
-        HIP_TRY(launch_vnl<T>(c, k, nb, psi, ld, hpsi, ldo, c.partial.ptr, c.partial.bytes, c.cbuf.ptr),
-                "nonlocal kernels");
+    return cap;
+}
+
+// H|psi> for nseg segments (k-points), kMaxSeg per launch.  launch = false
+// only allocates (projector matrices, workspace, partial sums), so that a
+// following launch = true pass enqueues kernels and nothing else.
+template <typename T>
+static int run_hpsi(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool acc, bool launch) {
+    using C = cx_t<T>;
+    int rc;
+    for (int i = 0; i < nseg; ++i)
+        if ((rc = ensure_proj<T>(c, *const_cast<KptDev*>(segs[i].k), false))) return rc;
+    const size_t cap = work_cap(c);
+    for (int g0 = 0; g0 < nseg; g0 += kMaxSeg) {
+        const int ng = std::min(kMaxSeg, nseg - g0);
+        const ApplySeg* sg = segs + g0;
+        const size_t wpb = std::max<size_t>(1, local_work_bytes<T>(c, sg, ng, 1));
+        const int chunk = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nb, cap / wpb, (size_t)65535}));
+        if ((rc = ensure(c, c.work, wpb * chunk, "FFT workspace"))) return rc;
+        if (c.nch() > 0) {
+            int mn = sg[0].k->npw;
+            for (int i = 1; i < ng; ++i) mn = std::min(mn, sg[i].k->npw);
+            if ((rc = ensure(c, c.partial, vnl_partial_bytes(c, c.nch(), nb, mn, ng, sizeof(C)),
+                             "projection partial sums")))
+                return rc;
+            if ((rc = ensure(c, c.cbuf, (size_t)ng * c.nch() * nb * sizeof(C), "projections"))) return rc;
+        }
+        if (!launch) continue;
+        for (int b0 = 0; b0 < nb; b0 += chunk) {
+            const int nbc = std::min(chunk, nb - b0);
+            HIP_TRY(launch_local<T>(c, sg, ng, b0, nbc, acc, c.work.ptr), "local potential kernels");
+        }
+        if (c.nch() > 0)
+            HIP_TRY(launch_vnl<T>(c, sg, ng, nb, c.partial.ptr, c.partial.bytes, c.cbuf.ptr), "nonlocal kernels");
     }
     return kOk;
 }
 
 template <typename T>
-static int run_dhdk(Ctx& c, KptDev& k, int nb, const void* psi, int64_t ld, void* ox, void* oy,
-                    void* oz, int64_t ldo, bool acc) {
+static int run_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool acc, bool launch) {
     using C = cx_t<T>;
-    int rc = ensure_proj<T>(c, k, true);
-    if (rc) return rc;
-    if (c.nch() > 0) {
-        size_t pb = vnl_partial_bytes(4 * c.nch(), nb, k.npw, sizeof(C));
-        if ((rc = ensure(c, c.partial, pb, "projection partial sums"))) return rc;
-        if ((rc = ensure(c, c.cbuf, (size_t)4 * c.nch() * nb * sizeof(C), "projections"))) return rc;
+    int rc;
+    for (int i = 0; i < nseg; ++i)
+        if ((rc = ensure_proj<T>(c, *const_cast<KptDev*>(segs[i].k), true))) return rc;
+    for (int g0 = 0; g0 < nseg; g0 += kMaxSeg) {
+        const int ng = std::min(kMaxSeg, nseg - g0);
+        const ApplySeg* sg = segs + g0;
+        if (c.nch() > 0) {
+            int mn = sg[0].k->npw;
+            for (int i = 1; i < ng; ++i) mn = std::min(mn, sg[i].k->npw);
+            if ((rc = ensure(c, c.partial, vnl_partial_bytes(c, 4 * c.nch(), nb, mn, ng, sizeof(C)),
+                             "projection partial sums")))
+                return rc;
+            if ((rc = ensure(c, c.cbuf, (size_t)ng * 4 * c.nch() * nb * sizeof(C), "projections"))) return rc;
+        }
+        if (!launch) continue;
+        HIP_TRY(launch_dhdk<T>(c, sg, ng, nb, acc, c.partial.ptr, c.partial.bytes, c.cbuf.ptr), "dH/dk kernels");
     }
-    HIP_TRY(launch_dhdk<T>(c, k, nb, psi, ld, ox, oy, oz, ldo, acc, c.partial.ptr, c.partial.bytes,
-                           c.cbuf.ptr),
-            "dH/dk kernels");
     return kOk;
+}
+
+// ---------------------------------------------------------------------------
+//  hipGraph capture / replay of an apply sequence (EFFM_F_GRAPH)
+// ---------------------------------------------------------------------------
+static constexpr size_t kMaxGraphs = 16;
+
+static void drop_graphs(Ctx& c) {
+    for (auto& g : c.graphs) (void)hipGraphExecDestroy(g.exec);
+    c.graphs.clear();
+}
+
+static std::vector<int64_t> graph_key(int kind, int prec, bool acc, int nb, const ApplySeg* segs, int nseg) {
+    std::vector<int64_t> key = {kind, prec, acc ? 1 : 0, nb, nseg};
+    for (int i = 0; i < nseg; ++i) {
+        key.push_back((int64_t)(intptr_t)segs[i].k);
+        key.push_back((int64_t)(intptr_t)segs[i].psi);
+        key.push_back(segs[i].ld);
+        for (int a = 0; a < 3; ++a) key.push_back((int64_t)(intptr_t)segs[i].out[a]);
+        key.push_back(segs[i].ldo);
+    }
+    return key;
+}
+
+// Runs `enqueue` (a launch = true pass) eagerly, or, with graph = true, as a
+// replay of a captured graph of the same launches (captured on first use).
+// Allocation happened before (launch = false pass), so the capture records
+// kernels only: no allocation, copy or memset nodes.
+template <typename F>
+static int run_maybe_graph(Ctx& c, bool graph, std::vector<int64_t> key, F&& enqueue) {
+    if (!graph) return enqueue();
+    if (c.graph_epoch != c.epoch) {
+        drop_graphs(c);
+        c.graph_epoch = c.epoch;
+    }
+    key.push_back((int64_t)(intptr_t)c.stream);
+    for (auto& g : c.graphs)
+        if (g.key == key) {
+            g.uses = ++c.graph_clock;
+            HIP_TRY(hipGraphLaunch(g.exec, c.stream), "graph launch");
+            return kOk;
+        }
+    if (!c.capture_stream) HIP_TRY(hipStreamCreateWithFlags(&c.capture_stream, hipStreamNonBlocking), "capture stream");
+    hipStream_t user = c.stream;
+    HIP_TRY(hipStreamBeginCapture(c.capture_stream, hipStreamCaptureModeRelaxed), "begin capture");
+    c.stream = c.capture_stream;
+    c.capturing = true;
+    int rc = enqueue();
+    c.capturing = false;
+    c.stream = user;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c.capture_stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "end capture");
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(c, e, "graph instantiate");
+    if (c.graphs.size() >= kMaxGraphs) {
+        auto lru = std::min_element(c.graphs.begin(), c.graphs.end(),
+                                    [](const Ctx::Graph& a, const Ctx::Graph& b) { return a.uses < b.uses; });
+        (void)hipGraphExecDestroy(lru->exec);
+        c.graphs.erase(lru);
+    }
+    c.graphs.push_back({std::move(key), exec, ++c.graph_clock});
+    HIP_TRY(hipGraphLaunch(exec, c.stream), "graph launch");
+    return kOk;
+}
+
+template <typename T>
+static int apply_hpsi(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool acc, bool graph, int prec) {
+    int rc = run_hpsi<T>(c, segs, nseg, nb, acc, false);
+    if (rc) return rc;
+    return run_maybe_graph(c, graph, graph ? graph_key(0, prec, acc, nb, segs, nseg) : std::vector<int64_t>{},
+                           [&] { return run_hpsi<T>(c, segs, nseg, nb, acc, true); });
+}
+
+template <typename T>
+static int apply_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool acc, bool graph, int prec) {
+    int rc = run_dhdk<T>(c, segs, nseg, nb, acc, false);
+    if (rc) return rc;
+    return run_maybe_graph(c, graph, graph ? graph_key(1, prec, acc, nb, segs, nseg) : std::vector<int64_t>{},
+                           [&] { return run_dhdk<T>(c, segs, nseg, nb, acc, true); });
 }
 
 }  // namespace effm
@@ -315,6 +429,8 @@ int effm_ctx_destroy(effm_ctx* h) {
         dfree(c, *b);
     timer_resolve(c);
     for (auto e : c.event_pool) (void)hipEventDestroy(e);
+    drop_graphs(c);
+    if (c.capture_stream) (void)hipStreamDestroy(c.capture_stream);
     delete h;
     return kOk;
 }
@@ -590,6 +706,8 @@ int effm_hpsi(effm_ctx* h, int ikpt, int nb, const void* psi, int64_t ld, void* 
     if (!psi || !hpsi) return fail(c, kErrInvalid, "psi / hpsi must not be NULL");
     if (ld < k.npw || ld_out < k.npw) return fail(c, kErrInvalid, "leading dimension smaller than npw");
     if (prec != EFFM_PREC_FP64 && prec != EFFM_PREC_MIXED) return fail(c, kErrInvalid, "unknown precision");
+    if ((flags & EFFM_F_HOST) && (flags & EFFM_F_GRAPH))
+        return fail(c, kErrInvalid, "EFFM_F_GRAPH takes device pointers only");
     const bool acc = !(flags & EFFM_F_OVERWRITE);
     const size_t esz = prec == EFFM_PREC_FP64 ? 16 : 8;
     const void* dpsi = psi;
@@ -600,8 +718,10 @@ int effm_hpsi(effm_ctx* h, int ikpt, int nb, const void* psi, int64_t ld, void* 
         dpsi = c.stage_in.ptr;
         dh = c.stage_out0.ptr;
     }
-    rc = prec == EFFM_PREC_FP64 ? run_hpsi<double>(c, k, nb, dpsi, ld, dh, ld_out, acc)
-                                : run_hpsi<float>(c, k, nb, dpsi, ld, dh, ld_out, acc);
+    ApplySeg seg{&k, dpsi, ld, {dh, nullptr, nullptr}, ld_out};
+    const bool graph = (flags & EFFM_F_GRAPH) != 0;
+    rc = prec == EFFM_PREC_FP64 ? apply_hpsi<double>(c, &seg, 1, nb, acc, graph, prec)
+                                : apply_hpsi<float>(c, &seg, 1, nb, acc, graph, prec);
     if (rc) return rc;
     if (flags & EFFM_F_HOST) {
         HIP_TRY(hipMemcpyAsync(hpsi, dh, esz * ld_out * nb, hipMemcpyDeviceToHost, c.stream), "copy hpsi");
@@ -623,6 +743,8 @@ int effm_dhdk(effm_ctx* h, int ikpt, int nb, const void* psi, int64_t ld, void* 
     if (!psi) return fail(c, kErrInvalid, "psi must not be NULL");
     if (ld < k.npw || ld_out < k.npw) return fail(c, kErrInvalid, "leading dimension smaller than npw");
     if (prec != EFFM_PREC_FP64 && prec != EFFM_PREC_MIXED) return fail(c, kErrInvalid, "unknown precision");
+    if ((flags & EFFM_F_HOST) && (flags & EFFM_F_GRAPH))
+        return fail(c, kErrInvalid, "EFFM_F_GRAPH takes device pointers only");
     const bool acc = !(flags & EFFM_F_OVERWRITE);
     const size_t esz = prec == EFFM_PREC_FP64 ? 16 : 8;
     const void* dpsi = psi;
@@ -638,8 +760,10 @@ int effm_dhdk(effm_ctx* h, int ikpt, int nb, const void* psi, int64_t ld, void* 
             douts[a] = st[a]->ptr;
         }
     }
-    rc = prec == EFFM_PREC_FP64 ? run_dhdk<double>(c, k, nb, dpsi, ld, douts[0], douts[1], douts[2], ld_out, acc)
-                                : run_dhdk<float>(c, k, nb, dpsi, ld, douts[0], douts[1], douts[2], ld_out, acc);
+    ApplySeg seg{&k, dpsi, ld, {douts[0], douts[1], douts[2]}, ld_out};
+    const bool graph = (flags & EFFM_F_GRAPH) != 0;
+    rc = prec == EFFM_PREC_FP64 ? apply_dhdk<double>(c, &seg, 1, nb, acc, graph, prec)
+                                : apply_dhdk<float>(c, &seg, 1, nb, acc, graph, prec);
     if (rc) return rc;
     if (flags & EFFM_F_HOST) {
         for (int a = 0; a < 3; ++a)
@@ -651,18 +775,59 @@ int effm_dhdk(effm_ctx* h, int ikpt, int nb, const void* psi, int64_t ld, void* 
     return kOk;
 }
 
+// Validates a multi-k batch and builds its segments.
+static int multi_segments(Ctx& c, int nk, const int* ikpts, int nb, const void* const* psi_ptrs,
+                          const int64_t* ld, void* const* o0, void* const* o1, void* const* o2, int prec,
+                          int flags, std::vector<ApplySeg>& segs) {
+    if (nk < 0 || (nk > 0 && (!ikpts || !psi_ptrs || !ld)))
+        return fail(c, kErrInvalid, "invalid batch description");
+    if (flags & EFFM_F_HOST) return fail(c, kErrInvalid, "batched applies take device pointers only");
+    if (nb < 0) return fail(c, kErrInvalid, "nb must be non-negative");
+    if (prec != EFFM_PREC_FP64 && prec != EFFM_PREC_MIXED) return fail(c, kErrInvalid, "unknown precision");
+    segs.clear();
+    for (int i = 0; i < nk; ++i) {
+        int rc = check_kpt(c, ikpts[i]);
+        if (rc) return rc;
+        const KptDev& k = c.kpts[ikpts[i]];
+        void* out[3] = {o0 ? o0[i] : nullptr, o1 ? o1[i] : nullptr, o2 ? o2[i] : nullptr};
+        if (!psi_ptrs[i]) return fail(c, kErrInvalid, "psi must not be NULL");
+        if (ld[i] < k.npw) return fail(c, kErrInvalid, "leading dimension smaller than npw");
+        segs.push_back(ApplySeg{&k, psi_ptrs[i], ld[i], {out[0], out[1], out[2]}, ld[i]});
+    }
+    return kOk;
+}
+
 int effm_hpsi_multi(effm_ctx* h, int nk, const int* ikpts, int nb, const void* const* psi_ptrs,
                     const int64_t* ld, void* const* hpsi_ptrs, int prec, int flags) {
     if (!h) return kErrInvalid;
     Ctx& c = h->c;
-    if (nk < 0 || (nk > 0 && (!ikpts || !psi_ptrs || !ld || !hpsi_ptrs)))
-        return fail(c, kErrInvalid, "invalid batch description");
-    if (flags & EFFM_F_HOST) return fail(c, kErrInvalid, "effm_hpsi_multi takes device pointers only");
-    for (int i = 0; i < nk; ++i) {
-        int rc = effm_hpsi(h, ikpts[i], nb, psi_ptrs[i], ld[i], hpsi_ptrs[i], ld[i], prec, flags);
-        if (rc) return rc;
-    }
-    return kOk;
+    (void)hipSetDevice(c.device);
+    std::vector<ApplySeg> segs;
+    int rc = multi_segments(c, nk, ikpts, nb, psi_ptrs, ld, hpsi_ptrs, nullptr, nullptr, prec, flags, segs);
+    if (rc) return rc;
+    if (nk > 0 && !hpsi_ptrs) return fail(c, kErrInvalid, "hpsi pointers must not be NULL");
+    for (const auto& s : segs)
+        if (!s.out[0]) return fail(c, kErrInvalid, "hpsi must not be NULL");
+    if (nk == 0 || nb == 0) return kOk;
+    const bool acc = !(flags & EFFM_F_OVERWRITE);
+    const bool graph = (flags & EFFM_F_GRAPH) != 0;
+    return prec == EFFM_PREC_FP64 ? apply_hpsi<double>(c, segs.data(), nk, nb, acc, graph, prec)
+                                  : apply_hpsi<float>(c, segs.data(), nk, nb, acc, graph, prec);
+}
+
+int effm_dhdk_multi(effm_ctx* h, int nk, const int* ikpts, int nb, const void* const* psi_ptrs,
+                    const int64_t* ld, void* const* ox, void* const* oy, void* const* oz, int prec, int flags) {
+    if (!h) return kErrInvalid;
+    Ctx& c = h->c;
+    (void)hipSetDevice(c.device);
+    std::vector<ApplySeg> segs;
+    int rc = multi_segments(c, nk, ikpts, nb, psi_ptrs, ld, ox, oy, oz, prec, flags, segs);
+    if (rc) return rc;
+    if (nk == 0 || nb == 0 || (!ox && !oy && !oz)) return kOk;
+    const bool acc = !(flags & EFFM_F_OVERWRITE);
+    const bool graph = (flags & EFFM_F_GRAPH) != 0;
+    return prec == EFFM_PREC_FP64 ? apply_dhdk<double>(c, segs.data(), nk, nb, acc, graph, prec)
+                                  : apply_dhdk<float>(c, segs.data(), nk, nb, acc, graph, prec);
 }
 
 int effm_rayleigh(effm_ctx* h, int npw, int nb, const void* psi, int64_t ld, const void* hpsi,

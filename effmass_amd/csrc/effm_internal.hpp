@@ -115,6 +115,20 @@ struct Ctx {
     // per kernel: resident workgroups per CU (dynamic LDS attribute set once)
     std::map<const void*, int> occupancy;
 
+    // captured apply sequences (EFFM_F_GRAPH), keyed by everything the
+    // launches read; any device (re)allocation bumps epoch and drops them
+    struct Graph {
+        std::vector<int64_t> key;
+        hipGraphExec_t exec;
+        uint64_t uses;
+    };
+    std::vector<Graph> graphs;
+    uint64_t epoch = 0;
+    uint64_t graph_epoch = 0;
+    uint64_t graph_clock = 0;
+    hipStream_t capture_stream = nullptr;
+    bool capturing = false;
+
     // scratch (grown on demand)
     DevBuf work;       // FFT plane workspace
     DevBuf partial;    // split-K partial sums
@@ -167,26 +181,45 @@ constexpr int64_t ws_plane(int n2, int n3) {
     return (int64_t)n2 * ws_pitch<T>(n3);
 }
 
+// ---- batched applies ------------------------------------------------------
+// One launch of each kernel covers up to kMaxSeg k-points ("segments") with
+// their own ragged G-sphere, plane CSR, projector matrices and psi / output
+// buffers; the per-segment descriptors travel as kernel arguments.
+constexpr int kMaxSeg = 16;
+
+// Host-side description of one k-point of a batched apply.
+struct ApplySeg {
+    const KptDev* k;
+    const void* psi;
+    int64_t ld;
+    void* out[3];         // H|psi>: out[0]; dH/dk: x, y, z (NULL = skip)
+    int64_t ldo;
+};
+
 // ---- launchers (k_local.hip) --------------------------------------------
+// bands [b0, b0 + nb) of every segment; work holds the segments' planes back
+// to back (local_work_bytes)
 template <typename T>
-hipError_t launch_local(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld,
-                        void* hpsi, int64_t ldo, bool accumulate, void* work);
+hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, bool accumulate,
+                        void* work);
+
+template <typename T>
+size_t local_work_bytes(const Ctx& c, const ApplySeg* segs, int nseg, int nb);
 
 // ---- launchers (k_nonlocal.hip) -----------------------------------------
 template <typename T>
 hipError_t launch_build_proj(Ctx& c, KptDev& k, bool grad);
 
 template <typename T>
-hipError_t launch_vnl(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld,
-                      void* hpsi, int64_t ldo, void* partial, size_t partial_bytes,
-                      void* cbuf);
+hipError_t launch_vnl(Ctx& c, const ApplySeg* segs, int nseg, int nb, void* partial,
+                      size_t partial_bytes, void* cbuf);
 
 template <typename T>
-hipError_t launch_dhdk(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld,
-                       void* ox, void* oy, void* oz, int64_t ldo, bool accumulate,
+hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accumulate,
                        void* partial, size_t partial_bytes, void* cbuf);
 
-size_t vnl_partial_bytes(int nrows, int nb, int npw, size_t elem);
+// split-K partial sums of nseg segments with nrows projector rows each
+size_t vnl_partial_bytes(const Ctx& c, int nrows, int nb, int min_npw, int nseg, size_t elem);
 
 template <typename T>
 hipError_t launch_rayleigh(hipStream_t s, int npw, int nb, const void* psi, int64_t ld,

@@ -4,20 +4,26 @@
 // G2R FFT, V_loc multiply, R2G FFT, gather) and src/math/fft.cppm:60-85.
 //
 // Three launches per band batch, each an HBM streaming pass:
-//   1. plane_g2r     one workgroup per (active x-plane, band): scatter the
+//   1. plane_g2r     one item per (active x-plane, band): scatter the
 //                    plane's G-sphere entries into a zeroed n2 x n3 LDS
 //                    plane at digit-reversed positions, DIT backward FFT
 //                    along k then j, write the plane.
-//   2. xpencil_vloc  one workgroup per (j, z-chunk, band): load the x-pencils
-//                    of the active planes (inactive x = 0), backward FFT
-//                    along x, multiply by V_loc(r), forward FFT along x,
-//                    store only the active planes (the R2G result is needed
-//                    only inside the sphere).
-//   3. plane_r2g     one workgroup per (active x-plane, band): forward FFT
-//                    along j and k (DIF), then gather the sphere entries:
+//   2. xpencil_vloc  one workgroup per (j, z-chunk, band group): load the
+//                    x-pencils of the active planes (inactive x = 0),
+//                    backward FFT along x, multiply by V_loc(r), forward FFT
+//                    along x, store only the active planes (the R2G result
+//                    is needed only inside the sphere).
+//   3. plane_r2g     one item per (active x-plane, band): forward FFT along
+//                    j and k (DIF), then gather the sphere entries:
 //                    hpsi = [hpsi +] kin*psi + V psi / N.
 // Planes outside the sphere's x-range are never written or read: this is
 // the zero-padding the reference's dense 3D FFT spends on them.
+//
+// Every launch covers a batch of up to kMaxSeg k-points ("segments"), each
+// with its own ragged G-sphere (npw), plane CSR, psi / hpsi and workspace;
+// the segment descriptors are kernel arguments.  The plane kernels walk the
+// concatenated (segment, plane, band) item list; the pencil kernel's grid
+// is (j, z-chunk) x (segment, band group).
 #include <algorithm>
 
 #include "effm_internal.hpp"
@@ -26,30 +32,66 @@
 namespace effm {
 
 template <typename T>
+struct LocalSeg {
+    const cx_t<T>* psi;       // band b0 of the batch
+    cx_t<T>* hpsi;
+    int64_t ld, ldo;
+    const int* plane_ptr;     // [nplanes + 1]
+    const int* ent_ig;        // [npw]
+    const int* ent_pos;       // [npw]
+    const int* x_to_plane;    // [n1]
+    const int* plane_x;       // [nplanes]
+    const double* kin;        // [npw]
+    cx_t<T>* work;            // [nb][nplanes][ws_plane]
+    int nplanes;
+    int item0;                // first (plane, band) item of the segment
+};
+
+template <typename T>
+struct LocalBatch {
+    int nseg;
+    int nb;                   // bands per segment
+    LocalSeg<T> s[kMaxSeg];
+};
+
+// Segment holding flat item `item` (segments are in ascending item0).
+template <typename T>
+__device__ __forceinline__ int seg_of(const LocalBatch<T>& bt, int item) {
+    int s = 0;
+    while (s + 1 < bt.nseg && item >= bt.s[s + 1].item0) ++s;
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+//  Generic kernels (any n1, n2, n3 the runtime radix set factors).
+//  Grids: planes (max nplanes, nb, nseg); pencils (n2 * z-chunks, nb, nseg).
+// ---------------------------------------------------------------------------
+template <typename T>
 __global__ void __launch_bounds__(512)
-k_plane_g2r(const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict__ work, int nplanes,
-            const int* __restrict__ plane_ptr, const int* __restrict__ ent_ig,
-            const int* __restrict__ ent_pos, FftAxis a2, FftAxis a3,
-            const cx_t<T>* __restrict__ tw2, const cx_t<T>* __restrict__ tw3, int wp) {
+k_plane_g2r(const LocalBatch<T> bt, FftAxis a2, FftAxis a3, const cx_t<T>* __restrict__ tw2,
+            const cx_t<T>* __restrict__ tw3) {
     using C = cx_t<T>;
+    const LocalSeg<T>& sg = bt.s[blockIdx.z];
+    const int p = blockIdx.x;
+    if (p >= sg.nplanes) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n2 = a2.n, n3 = a3.n, n23 = n2 * n3;
+    const int wp = ws_pitch<T>(n3);
     C* plane = reinterpret_cast<C*>(smem_raw);
     C* t2 = plane + n23;
     C* t3 = t2 + n2;
-    const int p = blockIdx.x;
     const int b = blockIdx.y;
     for (int i = threadIdx.x; i < n23; i += blockDim.x) plane[i] = C{0, 0};
     for (int i = threadIdx.x; i < n2; i += blockDim.x) t2[i] = tw2[i];
     for (int i = threadIdx.x; i < n3; i += blockDim.x) t3[i] = tw3[i];
     __syncthreads();
-    const C* ps = psi + (int64_t)b * ld;
-    const int e0 = plane_ptr[p], e1 = plane_ptr[p + 1];
-    for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) plane[ent_pos[e]] = ps[ent_ig[e]];
+    const C* ps = sg.psi + (int64_t)b * sg.ld;
+    const int e0 = sg.plane_ptr[p], e1 = sg.plane_ptr[p + 1];
+    for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) plane[sg.ent_pos[e]] = ps[sg.ent_ig[e]];
     __syncthreads();
     fft_lds<+1, true, T>(plane, a3, 1, n3, n2, t3);    // along k (rows)
     fft_lds<+1, true, T>(plane, a2, n3, 1, n3, t2);    // along j (columns)
-    C* out = work + ((int64_t)b * nplanes + p) * ws_plane<T>(n2, n3);
+    C* out = sg.work + ((int64_t)b * sg.nplanes + p) * ws_plane<T>(n2, n3);
     for (int i = threadIdx.x; i < n23; i += blockDim.x) {
         int j = i / n3;
         out[j * wp + (i - j * n3)] = plane[i];
@@ -58,11 +100,10 @@ k_plane_g2r(const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict__ w
 
 template <typename T, int TZ>
 __global__ void __launch_bounds__(256)
-k_xpencil_vloc(cx_t<T>* __restrict__ work, int nplanes, const int* __restrict__ plane_x,
-               const int* __restrict__ x_to_plane, const int* __restrict__ perm1,
-               const T* __restrict__ vloc, FftAxis a1, int n2, int n3,
-               const cx_t<T>* __restrict__ tw1, int wp) {
+k_xpencil_vloc(const LocalBatch<T> bt, const int* __restrict__ perm1, const T* __restrict__ vloc,
+               FftAxis a1, int n2, int n3, const cx_t<T>* __restrict__ tw1) {
     using C = cx_t<T>;
+    const LocalSeg<T>& sg = bt.s[blockIdx.z];
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n1 = a1.n;
     C* buf = reinterpret_cast<C*>(smem_raw);      // [n1][TZ]
@@ -72,13 +113,14 @@ k_xpencil_vloc(cx_t<T>* __restrict__ work, int nplanes, const int* __restrict__ 
     const int z0 = (blockIdx.x - j * nzc) * TZ;
     const int nz = min(TZ, n3 - z0);
     const int b = blockIdx.y;
+    const int nplanes = sg.nplanes;
     const int64_t n23 = (int64_t)n2 * n3;
     const int64_t wps = ws_plane<T>(n2, n3);   // workspace plane stride
-    cx_t<T>* wb = work + (int64_t)b * nplanes * wps + (int64_t)j * wp + z0;
+    cx_t<T>* wb = sg.work + (int64_t)b * nplanes * wps + (int64_t)j * ws_pitch<T>(n3) + z0;
     for (int i = threadIdx.x; i < n1; i += blockDim.x) t1[i] = tw1[i];
     for (int idx = threadIdx.x; idx < n1 * TZ; idx += blockDim.x) {
         int x = idx / TZ, z = idx - x * TZ;
-        int pl = x_to_plane[x];
+        int pl = sg.x_to_plane[x];
         C v = C{0, 0};
         if (pl >= 0 && z < nz) v = wb[(int64_t)pl * wps + z];
         buf[perm1[x] * TZ + z] = v;
@@ -98,26 +140,26 @@ k_xpencil_vloc(cx_t<T>* __restrict__ work, int nplanes, const int* __restrict__ 
     fft_lds<-1, false, T>(buf, a1, TZ, 1, TZ, t1);     // R2G along x
     for (int idx = threadIdx.x; idx < nplanes * TZ; idx += blockDim.x) {
         int pl = idx / TZ, z = idx - pl * TZ;
-        if (z < nz) wb[(int64_t)pl * wps + z] = buf[perm1[plane_x[pl]] * TZ + z];
+        if (z < nz) wb[(int64_t)pl * wps + z] = buf[perm1[sg.plane_x[pl]] * TZ + z];
     }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(512)
-k_plane_r2g(const cx_t<T>* __restrict__ work, int nplanes, const int* __restrict__ plane_ptr,
-            const int* __restrict__ ent_ig, const int* __restrict__ ent_pos,
-            const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict__ hpsi, int64_t ldo,
-            const double* __restrict__ kin, T scale, int accumulate, FftAxis a2, FftAxis a3,
-            const cx_t<T>* __restrict__ tw2, const cx_t<T>* __restrict__ tw3, int wp) {
+k_plane_r2g(const LocalBatch<T> bt, T scale, int accumulate, FftAxis a2, FftAxis a3,
+            const cx_t<T>* __restrict__ tw2, const cx_t<T>* __restrict__ tw3) {
     using C = cx_t<T>;
+    const LocalSeg<T>& sg = bt.s[blockIdx.z];
+    const int p = blockIdx.x;
+    if (p >= sg.nplanes) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n2 = a2.n, n3 = a3.n, n23 = n2 * n3;
+    const int wp = ws_pitch<T>(n3);
     C* plane = reinterpret_cast<C*>(smem_raw);
     C* t2 = plane + n23;
     C* t3 = t2 + n2;
-    const int p = blockIdx.x;
     const int b = blockIdx.y;
-    const C* in = work + ((int64_t)b * nplanes + p) * ws_plane<T>(n2, n3);
+    const C* in = sg.work + ((int64_t)b * sg.nplanes + p) * ws_plane<T>(n2, n3);
     for (int i = threadIdx.x; i < n23; i += blockDim.x) {
         int j = i / n3;
         plane[i] = in[j * wp + (i - j * n3)];
@@ -127,14 +169,14 @@ k_plane_r2g(const cx_t<T>* __restrict__ work, int nplanes, const int* __restrict
     __syncthreads();
     fft_lds<-1, false, T>(plane, a2, n3, 1, n3, t2);   // along j
     fft_lds<-1, false, T>(plane, a3, 1, n3, n2, t3);   // along k
-    const C* ps = psi + (int64_t)b * ld;
-    C* hs = hpsi + (int64_t)b * ldo;
-    const int e0 = plane_ptr[p], e1 = plane_ptr[p + 1];
+    const C* ps = sg.psi + (int64_t)b * sg.ld;
+    C* hs = sg.hpsi + (int64_t)b * sg.ldo;
+    const int e0 = sg.plane_ptr[p], e1 = sg.plane_ptr[p + 1];
     for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-        int ig = ent_ig[e];
-        C v = plane[ent_pos[e]];
+        int ig = sg.ent_ig[e];
+        C v = plane[sg.ent_pos[e]];
         C s = ps[ig];
-        double kn = kin[ig];
+        double kn = sg.kin[ig];
         // kinetic term in FP64 (callable.cpp:119-126), then the gathered
         // local-potential term (callable.cpp:171-182)
         double re = kn * (double)s.x + (double)(v.x * scale);
@@ -152,18 +194,17 @@ k_plane_r2g(const cx_t<T>* __restrict__ work, int nplanes, const int* __restrict
 //  Specialised kernels for square n2 = n3 = N planes and N1-point x pencils
 //  with compile-time two-stage plans (fft_ct.hpp).  Same data flow as above.
 // ---------------------------------------------------------------------------
-// Persistent plane kernels: each workgroup walks items = (plane, band) with
-// stride gridDim.x.  In FP32 the next item's global loads are issued before
-// this item's global stores, so both drain behind the next item's LDS work;
-// in FP64 the registers that would hold them are taken by the FFT itself.
+// Persistent plane kernels: each workgroup walks the flat (segment, plane,
+// band) items with stride gridDim.x.  In FP32 the next item's global loads
+// are issued before this item's global stores, so both drain behind the
+// next item's LDS work; in FP64 the registers that would hold them are
+// taken by the FFT itself.
 template <typename T>
 constexpr bool kPipePlanes = sizeof(T) == 4;
 
 template <typename T, int N, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-k_plane_g2r_ct(const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict__ work, int nplanes,
-               int nitems, const int* __restrict__ plane_ptr, const int* __restrict__ ent_ig,
-               const int* __restrict__ ent_pos, const cx_t<T>* __restrict__ tw_g) {
+k_plane_g2r_ct(const LocalBatch<T> bt, int nitems, const cx_t<T>* __restrict__ tw_g) {
     using C = cx_t<T>;
     constexpr int N2 = N * N;
     constexpr int WP = ws_pitch<T>(N);
@@ -181,23 +222,27 @@ k_plane_g2r_ct(const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict_
     int pp[PF];
     int e0 = 0, e1 = 0;
     auto prefetch = [&](int it) {
-        const int p = it % nplanes, b = it / nplanes;
-        e0 = plane_ptr[p];
-        e1 = plane_ptr[p + 1];
+        const LocalSeg<T>& sg = bt.s[seg_of(bt, it)];
+        const int local = it - sg.item0;
+        const int p = local % sg.nplanes, b = local / sg.nplanes;
+        e0 = sg.plane_ptr[p];
+        e1 = sg.plane_ptr[p + 1];
 #pragma unroll
         for (int f = 0; f < NPF; ++f) {
             const int e = e0 + threadIdx.x + f * BLOCK;
             pp[f] = -1;
             if (e < e1) {
-                pp[f] = ent_pos[e];
-                pv[f] = psi[(int64_t)b * ld + ent_ig[e]];
+                pp[f] = sg.ent_pos[e];
+                pv[f] = sg.psi[(int64_t)b * sg.ld + sg.ent_ig[e]];
             }
         }
     };
     int item = blockIdx.x;
     if (item < nitems) prefetch(item);
     for (; item < nitems; item += gridDim.x) {
-        const int p = item % nplanes, b = item / nplanes;
+        const LocalSeg<T>& sg = bt.s[seg_of(bt, item)];
+        const int local = item - sg.item0;
+        const int p = local % sg.nplanes, b = local / sg.nplanes;
         constexpr int NZ = (N2 + BLOCK - 1) / BLOCK;
 #pragma unroll
         for (int it = 0; it < NZ; ++it) {
@@ -209,11 +254,11 @@ k_plane_g2r_ct(const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict_
         for (int f = 0; f < NPF; ++f)
             if (pp[f] >= 0) plane[pp[f]] = pv[f];
         for (int e = e0 + threadIdx.x + NPF * BLOCK; e < e1; e += BLOCK)
-            plane[ent_pos[e]] = psi[(int64_t)b * ld + ent_ig[e]];
+            plane[sg.ent_pos[e]] = sg.psi[(int64_t)b * sg.ld + sg.ent_ig[e]];
         __syncthreads();
         ct_fft<N, +1, true, 1, N, N, BLOCK, T>(plane, tw);     // along k (rows)
         ct_fft<N, +1, true, N, 1, N, BLOCK, T>(plane, tw);     // along j (columns)
-        C* o = work + ((int64_t)b * nplanes + p) * PS;
+        C* o = sg.work + ((int64_t)b * sg.nplanes + p) * PS;
         const int nxt = item + gridDim.x;
         if constexpr (kPipePlanes<T>) {
             // FP32: plane -> registers, then the next item's loads, then the stores
@@ -250,11 +295,8 @@ k_plane_g2r_ct(const cx_t<T>* __restrict__ psi, int64_t ld, cx_t<T>* __restrict_
 
 template <typename T, int N, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-k_plane_r2g_ct(const cx_t<T>* __restrict__ work, int nplanes, int nitems,
-               const int* __restrict__ plane_ptr, const int* __restrict__ ent_ig,
-               const int* __restrict__ ent_pos, const cx_t<T>* __restrict__ psi, int64_t ld,
-               cx_t<T>* __restrict__ hpsi, int64_t ldo, const double* __restrict__ kin, T scale,
-               int accumulate, const cx_t<T>* __restrict__ tw_g) {
+k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
+               const cx_t<T>* __restrict__ tw_g) {
     using C = cx_t<T>;
     constexpr int N2 = N * N;
     constexpr int WP = ws_pitch<T>(N);
@@ -267,8 +309,10 @@ k_plane_r2g_ct(const cx_t<T>* __restrict__ work, int nplanes, int nitems,
     C* tw = plane + N2;
     if (threadIdx.x < N) tw[threadIdx.x] = tw_g[threadIdx.x];
     auto plane_in = [&](int it) {
-        const int p = it % nplanes, b = it / nplanes;
-        return work + ((int64_t)b * nplanes + p) * PS;
+        const LocalSeg<T>& sg = bt.s[seg_of(bt, it)];
+        const int local = it - sg.item0;
+        const int p = local % sg.nplanes, b = local / sg.nplanes;
+        return static_cast<const C*>(sg.work) + ((int64_t)b * sg.nplanes + p) * PS;
     };
     // FP32: the next plane is prefetched into registers behind this one's
     // FFT; FP64 loads each plane at the top of its item
@@ -288,24 +332,13 @@ k_plane_r2g_ct(const cx_t<T>* __restrict__ work, int nplanes, int nitems,
         const int kk = idx - j * WP;
         if (kk < N && j < N) plane[j * N + kk] = v;
     };
-    auto gather = [&](int e, const C* ps, C* hs) {
-        const int ig = ent_ig[e];
-        const C v = plane[ent_pos[e]];
-        const C sv = ps[ig];
-        double re = kin[ig] * (double)sv.x + (double)(v.x * scale);
-        double im = kin[ig] * (double)sv.y + (double)(v.y * scale);
-        if (accumulate) {
-            const C h = hs[ig];
-            re += (double)h.x;
-            im += (double)h.y;
-        }
-        hs[ig] = C{(T)re, (T)im};
-    };
     int item = blockIdx.x;
     if constexpr (kPipePlanes<T>)
         if (item < nitems) prefetch(item);
     for (; item < nitems; item += gridDim.x) {
-        const int p = item % nplanes, b = item / nplanes;
+        const LocalSeg<T>& sg = bt.s[seg_of(bt, item)];
+        const int local = item - sg.item0;
+        const int p = local % sg.nplanes, b = local / sg.nplanes;
         if constexpr (kPipePlanes<T>) {
 #pragma unroll
             for (int q = 0; q < NIN; ++q) to_lds(q, iv[q]);
@@ -315,9 +348,25 @@ k_plane_r2g_ct(const cx_t<T>* __restrict__ work, int nplanes, int nitems,
             for (int q = 0; q < NIN; ++q) to_lds(q, in[min((int)threadIdx.x + q * BLOCK, NP - 1)]);
         }
         __syncthreads();
-        const int e0 = plane_ptr[p], e1 = plane_ptr[p + 1];
-        const C* ps = psi + (int64_t)b * ld;
-        C* hs = hpsi + (int64_t)b * ldo;
+        const int e0 = sg.plane_ptr[p], e1 = sg.plane_ptr[p + 1];
+        const C* ps = sg.psi + (int64_t)b * sg.ld;
+        C* hs = sg.hpsi + (int64_t)b * sg.ldo;
+        const int* ent_ig = sg.ent_ig;
+        const int* ent_pos = sg.ent_pos;
+        const double* kin = sg.kin;
+        auto gather = [&](int e) {
+            const int ig = ent_ig[e];
+            const C v = plane[ent_pos[e]];
+            const C sv = ps[ig];
+            double re = kin[ig] * (double)sv.x + (double)(v.x * scale);
+            double im = kin[ig] * (double)sv.y + (double)(v.y * scale);
+            if (accumulate) {
+                const C h = hs[ig];
+                re += (double)h.x;
+                im += (double)h.y;
+            }
+            hs[ig] = C{(T)re, (T)im};
+        };
         if constexpr (kPipePlanes<T>) {
             // this item's gather inputs, then the next plane, both in flight
             // during the FFT
@@ -348,18 +397,18 @@ k_plane_r2g_ct(const cx_t<T>* __restrict__ work, int nplanes, int nitems,
                 double im = gk[f] * (double)gs[f].y + (double)(v.y * scale) + (double)gh[f].y;
                 hs[gi[f]] = C{(T)re, (T)im};
             }
-            for (int e = e0 + threadIdx.x + PF * BLOCK; e < e1; e += BLOCK) gather(e, ps, hs);
+            for (int e = e0 + threadIdx.x + PF * BLOCK; e < e1; e += BLOCK) gather(e);
         } else {
             ct_fft<N, -1, false, N, 1, N, BLOCK, T>(plane, tw);    // along j
             ct_fft<N, -1, false, 1, N, N, BLOCK, T>(plane, tw);    // along k
-            for (int e = e0 + threadIdx.x; e < e1; e += BLOCK) gather(e, ps, hs);
+            for (int e = e0 + threadIdx.x; e < e1; e += BLOCK) gather(e);
         }
         __syncthreads();
     }
 }
 
-// x pencils of one (j, z-chunk) for NBW consecutive bands, as register
-// FFTs.  N1 = P1 * P2 (Plan2).  Thread (z, role):
+// x pencils of one (j, z-chunk) for NBW consecutive bands of one segment,
+// as register FFTs.  N1 = P1 * P2 (Plan2).  Thread (z, role):
 //   G2R (DIT)  role b < P2 loads x = P2 u + b (u < P1) straight from the
 //              plane workspace, DFT_P1 -> LDS transpose -> role k1 < P1
 //              gathers Z[u'][k1], twiddle W^(u' k1), DFT_P2: it now holds
@@ -370,15 +419,22 @@ k_plane_r2g_ct(const cx_t<T>* __restrict__ work, int nplanes, int nitems,
 //              the active planes only.
 // Two LDS round trips per band (double-buffered transposes, two barriers);
 // the next band's loads are issued before the current band's arithmetic.
+// Grid: (n2 * z-chunks, nseg * band groups).
 template <typename T, int N1, int TZ, int NBW>
 __global__ void __launch_bounds__(TZ * cmax(Plan2<N1>::P1, Plan2<N1>::P2))
-k_xpencil_vloc_ct(cx_t<T>* __restrict__ work, int nplanes, int nb, const int* __restrict__ x_to_plane,
-                  const T* __restrict__ vloc, int n2, int n3, const cx_t<T>* __restrict__ tw_g, int wp) {
+k_xpencil_vloc_ct(const LocalBatch<T> bt, const T* __restrict__ vloc, int n2, int n3,
+                  const cx_t<T>* __restrict__ tw_g) {
     using C = cx_t<T>;
     constexpr int P1 = Plan2<N1>::P1, P2 = Plan2<N1>::P2;
     __shared__ C ex[2][N1 * TZ];
     __shared__ C tw[N1];
     __shared__ int srcpl[N1];
+    const int nbg = (bt.nb + NBW - 1) / NBW;
+    const int sidx = blockIdx.y / nbg;
+    const int bgrp = blockIdx.y - sidx * nbg;
+    const LocalSeg<T>& sg = bt.s[sidx];
+    const int nplanes = sg.nplanes;
+    C* work = sg.work;
     const int nzc = (n3 + TZ - 1) / TZ;
     const int j = blockIdx.x / nzc;
     const int z0 = (blockIdx.x - j * nzc) * TZ;
@@ -388,7 +444,7 @@ k_xpencil_vloc_ct(cx_t<T>* __restrict__ work, int nplanes, int nb, const int* __
     const int64_t n23 = (int64_t)n2 * n3;
     for (int i = threadIdx.x; i < N1; i += TZ * cmax(P1, P2)) {
         tw[i] = tw_g[i];
-        srcpl[i] = x_to_plane[i];
+        srcpl[i] = sg.x_to_plane[i];
     }
     // V at the natural x = role + P1 q' this thread scales
     T vr[P2];
@@ -399,7 +455,7 @@ k_xpencil_vloc_ct(cx_t<T>* __restrict__ work, int nplanes, int nb, const int* __
     }
     __syncthreads();
     const int64_t wps = ws_plane<T>(n2, n3);           // workspace plane stride
-    const int64_t col = (int64_t)j * wp + z0 + z;
+    const int64_t col = (int64_t)j * ws_pitch<T>(n3) + z0 + z;
     C y[P1];
     auto load = [&](int b) {
         const C* wb = work + (int64_t)b * nplanes * wps + col;
@@ -410,8 +466,8 @@ k_xpencil_vloc_ct(cx_t<T>* __restrict__ work, int nplanes, int nb, const int* __
             y[u] = (pl >= 0 && zv) ? wb[(int64_t)pl * wps] : C{0, 0};
         }
     };
-    const int bfirst = blockIdx.y * NBW;
-    const int bend = min(nb, bfirst + NBW);
+    const int bfirst = bgrp * NBW;
+    const int bend = min(bt.nb, bfirst + NBW);
     if (bfirst < bend) load(bfirst);
     for (int b = bfirst; b < bend; ++b) {
         C v[P1];
@@ -482,9 +538,9 @@ static bool plan_is(const FftAxis& a, int p1, int p2) {
 }
 
 // Persistent grid: as many workgroups as the occupancy calculator says are
-// resident at once (LDS, VGPRs, wave slots), capped by the (plane, band)
-// items.  The dynamic-LDS attribute and the occupancy are set up once per
-// kernel and context.
+// resident at once (LDS, VGPRs, wave slots), capped by the items.  The
+// dynamic-LDS attribute and the occupancy are set up once per kernel and
+// context.
 static hipError_t persistent_grid(Ctx& c, const void* fn, size_t lds, int block, int nitems, int* grid) {
     auto it = c.occupancy.find(fn);
     if (it == c.occupancy.end()) {
@@ -500,50 +556,38 @@ static hipError_t persistent_grid(Ctx& c, const void* fn, size_t lds, int block,
 }
 
 template <typename T, int N>
-static hipError_t launch_planes_ct(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld,
-                                   void* hpsi, int64_t ldo, bool accumulate, void* work, bool g2r,
-                                   const cx_t<T>* tw) {
-    using C = cx_t<T>;
+static hipError_t launch_planes_ct(Ctx& c, const LocalBatch<T>& bt, int nitems, bool accumulate,
+                                   bool g2r, const cx_t<T>* tw) {
     constexpr int BLOCK = plane_block<N>();
-    const size_t lds = sizeof(C) * ((size_t)N * N + N);
-    const int nitems = k.nplanes * nb;
+    const size_t lds = sizeof(cx_t<T>) * ((size_t)N * N + N);
     hipError_t e;
+    int grid = 0;
     if (g2r) {
-        int grid = 0;
         e = persistent_grid(c, reinterpret_cast<const void*>(&k_plane_g2r_ct<T, N, BLOCK>), lds, BLOCK,
                             nitems, &grid);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_plane_g2r_ct<T, N, BLOCK>), dim3(grid), dim3(BLOCK), lds, c.stream,
-                           static_cast<const C*>(psi), ld, static_cast<C*>(work), k.nplanes, nitems,
-                           static_cast<const int*>(k.plane_ptr.ptr), static_cast<const int*>(k.ent_ig.ptr),
-                           static_cast<const int*>(k.ent_pos.ptr), tw);
+        hipLaunchKernelGGL((k_plane_g2r_ct<T, N, BLOCK>), dim3(grid), dim3(BLOCK), lds, c.stream, bt,
+                           nitems, tw);
     } else {
-        int grid = 0;
         e = persistent_grid(c, reinterpret_cast<const void*>(&k_plane_r2g_ct<T, N, BLOCK>), lds, BLOCK,
                             nitems, &grid);
         if (e != hipSuccess) return e;
         T scale = (T)(1.0 / ((double)c.n1 * c.n2 * c.n3));
-        hipLaunchKernelGGL((k_plane_r2g_ct<T, N, BLOCK>), dim3(grid), dim3(BLOCK), lds, c.stream,
-                           static_cast<const C*>(work), k.nplanes, nitems,
-                           static_cast<const int*>(k.plane_ptr.ptr), static_cast<const int*>(k.ent_ig.ptr),
-                           static_cast<const int*>(k.ent_pos.ptr), static_cast<const C*>(psi), ld,
-                           static_cast<C*>(hpsi), ldo, static_cast<const double*>(k.kin.ptr), scale,
-                           accumulate ? 1 : 0, tw);
+        hipLaunchKernelGGL((k_plane_r2g_ct<T, N, BLOCK>), dim3(grid), dim3(BLOCK), lds, c.stream, bt,
+                           nitems, scale, accumulate ? 1 : 0, tw);
     }
     return hipGetLastError();
 }
 
 template <typename T, int N>
-static void launch_pencil_ct(Ctx& c, const KptDev& k, int nb, void* work, const T* vl,
-                             const cx_t<T>* tw1) {
-    using C = cx_t<T>;
+static void launch_pencil_ct(Ctx& c, const LocalBatch<T>& bt, const T* vl, const cx_t<T>* tw1) {
     constexpr int TZ = PencilShape<T, N>::TZ;
     constexpr int BLOCK = TZ * cmax(Plan2<N>::P1, Plan2<N>::P2);
-    int nzc = (c.n3 + TZ - 1) / TZ;
-    dim3 gx(c.n2 * nzc, (nb + kBandsPerPencilWG - 1) / kBandsPerPencilWG);
-    hipLaunchKernelGGL((k_xpencil_vloc_ct<T, N, TZ, kBandsPerPencilWG>), gx, dim3(BLOCK), 0, c.stream,
-                       static_cast<C*>(work), k.nplanes, nb, static_cast<const int*>(k.x_to_plane.ptr),
-                       vl, c.n2, c.n3, tw1, ws_pitch<T>(c.n3));
+    const int nzc = (c.n3 + TZ - 1) / TZ;
+    const int nbg = (bt.nb + kBandsPerPencilWG - 1) / kBandsPerPencilWG;
+    dim3 gx(c.n2 * nzc, nbg * bt.nseg);
+    hipLaunchKernelGGL((k_xpencil_vloc_ct<T, N, TZ, kBandsPerPencilWG>), gx, dim3(BLOCK), 0, c.stream, bt,
+                       vl, c.n2, c.n3, tw1);
 }
 
 template <typename T>
@@ -552,10 +596,45 @@ static size_t plane_lds_bytes(const Ctx& c) {
 }
 
 template <typename T>
-hipError_t launch_local(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld, void* hpsi,
-                        int64_t ldo, bool accumulate, void* work) {
+size_t local_work_bytes(const Ctx& c, const ApplySeg* segs, int nseg, int nb) {
+    size_t tot = 0;
+    for (int i = 0; i < nseg; ++i)
+        tot += (size_t)segs[i].k->nplanes * ws_plane<T>(c.n2, c.n3) * nb * sizeof(cx_t<T>);
+    return tot;
+}
+
+template <typename T>
+hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, bool accumulate,
+                        void* work) {
     using C = cx_t<T>;
     constexpr int TZ = 16;
+    if (nseg < 1 || nseg > kMaxSeg || nb < 1) return hipErrorInvalidValue;
+    LocalBatch<T> bt{};
+    bt.nseg = nseg;
+    bt.nb = nb;
+    int nitems = 0, maxplanes = 0;
+    C* wcur = static_cast<C*>(work);
+    for (int i = 0; i < nseg; ++i) {
+        const KptDev& k = *segs[i].k;
+        LocalSeg<T>& s = bt.s[i];
+        s.psi = static_cast<const C*>(segs[i].psi) + (int64_t)b0 * segs[i].ld;
+        s.hpsi = static_cast<C*>(segs[i].out[0]) + (int64_t)b0 * segs[i].ldo;
+        s.ld = segs[i].ld;
+        s.ldo = segs[i].ldo;
+        s.plane_ptr = static_cast<const int*>(k.plane_ptr.ptr);
+        s.ent_ig = static_cast<const int*>(k.ent_ig.ptr);
+        s.ent_pos = static_cast<const int*>(k.ent_pos.ptr);
+        s.x_to_plane = static_cast<const int*>(k.x_to_plane.ptr);
+        s.plane_x = static_cast<const int*>(k.plane_x.ptr);
+        s.kin = static_cast<const double*>(k.kin.ptr);
+        s.work = wcur;
+        s.nplanes = k.nplanes;
+        s.item0 = nitems;
+        wcur += (int64_t)k.nplanes * ws_plane<T>(c.n2, c.n3) * nb;
+        nitems += k.nplanes * nb;
+        maxplanes = std::max(maxplanes, k.nplanes);
+    }
+    if (nitems == 0) return hipSuccess;
     const bool f64 = sizeof(T) == 8;
     const C* tw1 = static_cast<const C*>(f64 ? c.tw1_64.ptr : c.tw1_32.ptr);
     const C* tw2 = static_cast<const C*>(f64 ? c.tw2_64.ptr : c.tw2_32.ptr);
@@ -564,15 +643,6 @@ hipError_t launch_local(Ctx& c, const KptDev& k, int nb, const void* psi, int64_
     const size_t lds_plane = plane_lds_bytes<T>(c);
     const size_t lds_pencil = sizeof(C) * ((size_t)c.n1 * TZ + c.n1);
     hipError_t e;
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_g2r<T>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_r2g<T>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_xpencil_vloc<T, TZ>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pencil);
-    if (e != hipSuccess) return e;
 
     // compile-time plans for the common square planes / pencil lengths
     const int np = c.n2;
@@ -584,60 +654,58 @@ hipError_t launch_local(Ctx& c, const KptDev& k, int nb, const void* psi, int64_
     auto ct_pencil = [&](int n) { return c.n1 == n && plan_is(c.ax1, Plan2N(n, 1), Plan2N(n, 2)); };
     const bool pl90 = ct_plane(90), pl48 = ct_plane(48), pl36 = ct_plane(36);
     const bool pe90 = ct_pencil(90), pe48 = ct_pencil(48), pe36 = ct_pencil(36);
-    if (pl90 || pl48 || pl36) {
-        timer_begin(c);
-        if (pl90) e = launch_planes_ct<T, 90>(c, k, nb, psi, ld, hpsi, ldo, accumulate, work, true, tw2);
-        else if (pl48) e = launch_planes_ct<T, 48>(c, k, nb, psi, ld, hpsi, ldo, accumulate, work, true, tw2);
-        else e = launch_planes_ct<T, 36>(c, k, nb, psi, ld, hpsi, ldo, accumulate, work, true, tw2);
-        timer_end(c, "plane_g2r");
+    const bool plct = pl90 || pl48 || pl36;
+    if (!plct || !(pe90 || pe48 || pe36)) {
+        // generic kernels: dynamic LDS sized by this cell's grid
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_g2r<T>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
         if (e != hipSuccess) return e;
-    } else {
-        dim3 gp(k.nplanes, nb);
-        timer_begin(c);
-        hipLaunchKernelGGL((k_plane_g2r<T>), gp, dim3(512), lds_plane, c.stream,
-                       static_cast<const C*>(psi), ld, static_cast<C*>(work), k.nplanes,
-                       static_cast<const int*>(k.plane_ptr.ptr), static_cast<const int*>(k.ent_ig.ptr),
-                       static_cast<const int*>(k.ent_pos.ptr), c.ax2, c.ax3, tw2, tw3, ws_pitch<T>(c.n3));
-        timer_end(c, "plane_g2r");
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_r2g<T>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_xpencil_vloc<T, TZ>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pencil);
+        if (e != hipSuccess) return e;
     }
+    const dim3 gp(maxplanes, nb, nseg);
     timer_begin(c);
-    if (pe90) launch_pencil_ct<T, 90>(c, k, nb, work, vl, tw1);
-    else if (pe48) launch_pencil_ct<T, 48>(c, k, nb, work, vl, tw1);
-    else if (pe36) launch_pencil_ct<T, 36>(c, k, nb, work, vl, tw1);
+    if (pl90) e = launch_planes_ct<T, 90>(c, bt, nitems, accumulate, true, tw2);
+    else if (pl48) e = launch_planes_ct<T, 48>(c, bt, nitems, accumulate, true, tw2);
+    else if (pl36) e = launch_planes_ct<T, 36>(c, bt, nitems, accumulate, true, tw2);
     else {
-        int nzc = (c.n3 + TZ - 1) / TZ;
-        dim3 gx(c.n2 * nzc, nb);
-        hipLaunchKernelGGL((k_xpencil_vloc<T, TZ>), gx, dim3(256), lds_pencil, c.stream,
-                           static_cast<C*>(work), k.nplanes, static_cast<const int*>(k.plane_x.ptr),
-                           static_cast<const int*>(k.x_to_plane.ptr), static_cast<const int*>(c.perm1.ptr),
-                           vl, c.ax1, c.n2, c.n3, tw1, ws_pitch<T>(c.n3));
+        hipLaunchKernelGGL((k_plane_g2r<T>), gp, dim3(512), lds_plane, c.stream, bt, c.ax2, c.ax3, tw2, tw3);
+        e = hipGetLastError();
+    }
+    timer_end(c, "plane_g2r");
+    if (e != hipSuccess) return e;
+    timer_begin(c);
+    if (pe90) launch_pencil_ct<T, 90>(c, bt, vl, tw1);
+    else if (pe48) launch_pencil_ct<T, 48>(c, bt, vl, tw1);
+    else if (pe36) launch_pencil_ct<T, 36>(c, bt, vl, tw1);
+    else {
+        const int nzc = (c.n3 + TZ - 1) / TZ;
+        hipLaunchKernelGGL((k_xpencil_vloc<T, TZ>), dim3(c.n2 * nzc, nb, nseg), dim3(256), lds_pencil, c.stream,
+                           bt, static_cast<const int*>(c.perm1.ptr), vl, c.ax1, c.n2, c.n3, tw1);
     }
     timer_end(c, "xpencil_vloc");
-    if (pl90 || pl48 || pl36) {
-        timer_begin(c);
-        if (pl90) e = launch_planes_ct<T, 90>(c, k, nb, psi, ld, hpsi, ldo, accumulate, work, false, tw2);
-        else if (pl48) e = launch_planes_ct<T, 48>(c, k, nb, psi, ld, hpsi, ldo, accumulate, work, false, tw2);
-        else e = launch_planes_ct<T, 36>(c, k, nb, psi, ld, hpsi, ldo, accumulate, work, false, tw2);
-        timer_end(c, "plane_r2g");
-        if (e != hipSuccess) return e;
-        return hipGetLastError();
-    }
-    dim3 gp(k.nplanes, nb);
-    T scale = (T)(1.0 / ((double)c.n1 * c.n2 * c.n3));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     timer_begin(c);
-    hipLaunchKernelGGL((k_plane_r2g<T>), gp, dim3(512), lds_plane, c.stream,
-                       static_cast<const C*>(work), k.nplanes,
-                       static_cast<const int*>(k.plane_ptr.ptr), static_cast<const int*>(k.ent_ig.ptr),
-                       static_cast<const int*>(k.ent_pos.ptr), static_cast<const C*>(psi), ld,
-                       static_cast<C*>(hpsi), ldo, static_cast<const double*>(k.kin.ptr), scale,
-                       accumulate ? 1 : 0, c.ax2, c.ax3, tw2, tw3, ws_pitch<T>(c.n3));
+    if (pl90) e = launch_planes_ct<T, 90>(c, bt, nitems, accumulate, false, tw2);
+    else if (pl48) e = launch_planes_ct<T, 48>(c, bt, nitems, accumulate, false, tw2);
+    else if (pl36) e = launch_planes_ct<T, 36>(c, bt, nitems, accumulate, false, tw2);
+    else {
+        T scale = (T)(1.0 / ((double)c.n1 * c.n2 * c.n3));
+        hipLaunchKernelGGL((k_plane_r2g<T>), gp, dim3(512), lds_plane, c.stream, bt, scale, accumulate ? 1 : 0,
+                           c.ax2, c.ax3, tw2, tw3);
+        e = hipGetLastError();
+    }
     timer_end(c, "plane_r2g");
-    return hipGetLastError();
+    return e;
 }
 
-template hipError_t launch_local<double>(Ctx&, const KptDev&, int, const void*, int64_t, void*,
-                                         int64_t, bool, void*);
-template hipError_t launch_local<float>(Ctx&, const KptDev&, int, const void*, int64_t, void*,
-                                        int64_t, bool, void*);
+template hipError_t launch_local<double>(Ctx&, const ApplySeg*, int, int, int, bool, void*);
+template hipError_t launch_local<float>(Ctx&, const ApplySeg*, int, int, int, bool, void*);
+template size_t local_work_bytes<double>(const Ctx&, const ApplySeg*, int, int);
+template size_t local_work_bytes<float>(const Ctx&, const ApplySeg*, int, int);
 
 }  // namespace effm

@@ -17,6 +17,8 @@
 // (src/pseudo/ncpp_advance.cppm:644-674), real Y_lm recurrences
 // (src/math/sph_harmonics.cppm:120-335, 702-875) and separable structure
 // factor (src/H_psi/structure_factor.cppm:92-159).
+#include <algorithm>
+
 #include "effm_internal.hpp"
 
 namespace effm {
@@ -249,17 +251,39 @@ __device__ inline int xcd_order(int i, int G) {
 //  slabs of A and B are fetched into that XCD's L2 once.
 // ---------------------------------------------------------------------------
 template <typename T>
+struct ProjSeg {
+    const cx_t<T>* A;      // [M][lda]
+    const cx_t<T>* B;      // [N][ldb]
+    cx_t<T>* part;         // [nsplit][Mtot][N]
+    int64_t lda, ldb;
+    int K, kchunk;
+};
+
+template <typename T>
+struct ProjBatch {
+    int nseg;
+    ProjSeg<T> s[kMaxSeg];
+};
+
+template <typename T>
 __global__ void __launch_bounds__(256)
-k_gemm_proj(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restrict__ B,
-            int64_t ldb, int M, int N, int K, int kchunk, cx_t<T>* __restrict__ part,
-            int Mtot, int row0) {
+k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0) {
     using C = cx_t<T>;
     using V = typename Acc<T>::type;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, kk = lane >> 4;
     const int nx = (M + 63) / 64, nt = nx * ((N + 63) / 64);
-    const int L = xcd_order(blockIdx.x, gridDim.x);
+    const int L0 = xcd_order(blockIdx.x, gridDim.x);
+    // segment-major, then split, then tile: one segment's splits stay together
+    const int seg = L0 / (nt * nsplit);
+    const int L = L0 - seg * nt * nsplit;
     const int split = L / nt, t = L - split * nt;
+    const ProjSeg<T>& sg = bt.s[seg];
+    const C* __restrict__ A = sg.A;
+    const C* __restrict__ B = sg.B;
+    C* __restrict__ part = sg.part;
+    const int64_t lda = sg.lda, ldb = sg.ldb;
+    const int K = sg.K, kchunk = sg.kchunk;
     const int m0 = (t % nx) * 64 + (wave & 1) * 32;
     const int n0 = (t / nx) * 64 + (wave >> 1) * 32;
     const int ks = split * kchunk;
@@ -349,11 +373,22 @@ k_gemm_proj(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restric
 
 // out[m][n] = lambda[m % nch] * sum_s part[s][m][n], summed in FP64.
 template <typename T>
-__global__ void k_reduce_scale(const cx_t<T>* __restrict__ part, int nsplit, int Mtot, int N,
-                               const double* __restrict__ lam, int nch, cx_t<T>* __restrict__ out) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+struct ReduceBatch {
+    int nseg;
+    const cx_t<T>* part[kMaxSeg];
+    cx_t<T>* out[kMaxSeg];
+};
+
+// grid: nseg x per_seg blocks of 256
+template <typename T>
+__global__ void k_reduce_scale(const ReduceBatch<T> bt, int per_seg, int nsplit, int Mtot, int N,
+                               const double* __restrict__ lam, int nch) {
+    const int seg = blockIdx.x / per_seg;
+    const int64_t idx = (int64_t)(blockIdx.x - seg * per_seg) * blockDim.x + threadIdx.x;
     const int64_t tot = (int64_t)Mtot * N;
     if (idx >= tot) return;
+    const cx_t<T>* __restrict__ part = bt.part[seg];
+    cx_t<T>* __restrict__ out = bt.out[seg];
     double re = 0.0, im = 0.0;
     for (int s = 0; s < nsplit; ++s) {
         cx_t<T> v = part[(int64_t)s * tot + idx];
@@ -372,17 +407,43 @@ __global__ void k_reduce_scale(const cx_t<T>* __restrict__ part, int nsplit, int
 //  each projector slab is fetched into one L2.
 // ---------------------------------------------------------------------------
 template <typename T>
+struct ApplySegK {
+    const cx_t<T>* A;      // [K][lda]
+    const cx_t<T>* B;      // [K][ldb]
+    cx_t<T>* out;          // [N][ldo]
+    const double* diag;    // [M] or NULL
+    const cx_t<T>* src;    // [N][lds] (with diag)
+    int64_t lda, ldb, ldo, lds;
+    int M;
+    int tile0;             // first tile of the segment
+};
+
+template <typename T>
+struct ApplyBatch {
+    int nseg;
+    ApplySegK<T> s[kMaxSeg];
+};
+
+template <typename T>
 __global__ void __launch_bounds__(256)
-k_gemm_apply(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restrict__ B,
-             int64_t ldb, int M, int N, int K, cx_t<T>* __restrict__ out, int64_t ldo,
-             int accumulate, const double* __restrict__ diag, const cx_t<T>* __restrict__ src,
-             int64_t lds) {
+k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     using C = cx_t<T>;
     using V = typename Acc<T>::type;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, kk = lane >> 4;
     const int ny = (N + 63) / 64;
-    const int L = xcd_order(blockIdx.x, gridDim.x);
+    const int L0 = xcd_order(blockIdx.x, gridDim.x);
+    int seg = 0;
+    while (seg + 1 < bt.nseg && L0 >= bt.s[seg + 1].tile0) ++seg;
+    const ApplySegK<T>& sg = bt.s[seg];
+    const int L = L0 - sg.tile0;
+    const C* __restrict__ A = sg.A;
+    const C* __restrict__ B = sg.B;
+    C* __restrict__ out = sg.out;
+    const double* __restrict__ diag = sg.diag;
+    const C* __restrict__ src = sg.src;
+    const int64_t lda = sg.lda, ldb = sg.ldb, ldo = sg.ldo, lds = sg.lds;
+    const int M = sg.M;
     const int m0 = (L / ny) * 64 + (wave & 1) * 32;
     const int n0 = (L % ny) * 64 + (wave >> 1) * 32;
     // Gauss's three-product complex MAC with a = Re P, b = Im P, c = Re C,
@@ -475,8 +536,9 @@ k_gemm_apply(const cx_t<T>* __restrict__ A, int64_t lda, const cx_t<T>* __restri
         }
 }
 
-static int choose_split(int M, int N, int K) {
-    int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+// Split-K count: about 1024 workgroups over the batch, chunks of >= 256 G.
+static int choose_split(int M, int N, int K, int nseg) {
+    int tiles = ((M + 63) / 64) * ((N + 63) / 64) * std::max(1, nseg);
     int want = (1024 + tiles - 1) / tiles;
     int maxs = (K + 255) / 256;
     if (want > maxs) want = maxs;
@@ -484,125 +546,176 @@ static int choose_split(int M, int N, int K) {
     return want;
 }
 
-size_t vnl_partial_bytes(int nrows, int nb, int npw, size_t elem) {
-    int ns = choose_split(nrows, nb, npw);
-    return (size_t)ns * nrows * nb * elem;
+size_t vnl_partial_bytes(const Ctx& c, int nrows, int nb, int min_npw, int nseg, size_t elem) {
+    (void)c;
+    int ns = choose_split(nrows, nb, min_npw, nseg);
+    return (size_t)ns * nrows * nb * elem * nseg;
 }
 
+static int min_npw(const ApplySeg* segs, int nseg) {
+    int m = segs[0].k->npw;
+    for (int i = 1; i < nseg; ++i) m = std::min(m, segs[i].k->npw);
+    return m;
+}
+
+// <A_s|psi_s> for every segment: rows [row0, row0 + M) of each segment's
+// partial buffer part_s = partial + s * nsplit * Mtot * nb.
 template <typename T>
-static hipError_t gemm_proj(Ctx& c, const cx_t<T>* A, int M, int npw, const cx_t<T>* psi,
-                            int64_t ld, int nb, cx_t<T>* part, int Mtot, int row0, int nsplit) {
-    int kchunk = (npw + nsplit - 1) / nsplit;
-    kchunk = (kchunk + 15) / 16 * 16;
-    dim3 grid(((M + 63) / 64) * ((nb + 63) / 64) * nsplit);
-    hipLaunchKernelGGL((k_gemm_proj<T>), grid, dim3(256), 0, c.stream, A, (int64_t)npw, psi, ld,
-                       M, nb, npw, kchunk, part, Mtot, row0);
+static hipError_t gemm_proj(Ctx& c, const ApplySeg* segs, int nseg, const cx_t<T>* const* A, int M,
+                            int nb, cx_t<T>* partial, int Mtot, int row0, int nsplit) {
+    ProjBatch<T> bt{};
+    bt.nseg = nseg;
+    for (int i = 0; i < nseg; ++i) {
+        const int npw = segs[i].k->npw;
+        int kchunk = (npw + nsplit - 1) / nsplit;
+        kchunk = (kchunk + 15) / 16 * 16;
+        bt.s[i] = ProjSeg<T>{A[i], static_cast<const cx_t<T>*>(segs[i].psi),
+                             partial + (size_t)i * nsplit * Mtot * nb, (int64_t)npw, segs[i].ld, npw, kchunk};
+    }
+    dim3 grid(((M + 63) / 64) * ((nb + 63) / 64) * nsplit * nseg);
+    hipLaunchKernelGGL((k_gemm_proj<T>), grid, dim3(256), 0, c.stream, bt, M, nb, nsplit, Mtot, row0);
     return hipGetLastError();
 }
 
 template <typename T>
-static hipError_t gemm_apply(Ctx& c, const cx_t<T>* A, int npw, const cx_t<T>* B, int nb,
-                             int K, cx_t<T>* out, int64_t ldo, bool acc, const double* diag,
-                             const cx_t<T>* src, int64_t lds) {
-    dim3 grid(((npw + 63) / 64) * ((nb + 63) / 64));
-    hipLaunchKernelGGL((k_gemm_apply<T>), grid, dim3(256), 0, c.stream, A, (int64_t)npw, B,
-                       (int64_t)nb, npw, nb, K, out, ldo, acc ? 1 : 0, diag, src, lds);
+static hipError_t reduce_scale(Ctx& c, int nseg, cx_t<T>* partial, int nsplit, int Mtot, int nb,
+                               cx_t<T>* cbuf) {
+    ReduceBatch<T> bt{};
+    bt.nseg = nseg;
+    for (int i = 0; i < nseg; ++i) {
+        bt.part[i] = partial + (size_t)i * nsplit * Mtot * nb;
+        bt.out[i] = cbuf + (size_t)i * Mtot * nb;
+    }
+    const int64_t tot = (int64_t)Mtot * nb;
+    const int per = (int)((tot + 255) / 256);
+    hipLaunchKernelGGL((k_reduce_scale<T>), dim3((unsigned)(per * nseg)), dim3(256), 0, c.stream, bt, per,
+                       nsplit, Mtot, nb, static_cast<const double*>(c.lam64.ptr), c.nch());
+    return hipGetLastError();
+}
+
+// out_s (+)= diag_s * src_s + A_s B_s for the segments with out_s != NULL
+template <typename T>
+static hipError_t gemm_apply(Ctx& c, int nseg, const ApplySegK<T>* in, int nb, int K, bool acc) {
+    ApplyBatch<T> bt{};
+    int tiles = 0;
+    const int ny = (nb + 63) / 64;
+    for (int i = 0; i < nseg; ++i) {
+        if (!in[i].out) continue;
+        ApplySegK<T>& s = bt.s[bt.nseg++];
+        s = in[i];
+        s.tile0 = tiles;
+        tiles += ((s.M + 63) / 64) * ny;
+    }
+    if (bt.nseg == 0 || tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_gemm_apply<T>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, acc ? 1 : 0);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_vnl(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld, void* hpsi,
-                      int64_t ldo, void* partial, size_t partial_bytes, void* cbuf) {
+hipError_t launch_vnl(Ctx& c, const ApplySeg* segs, int nseg, int nb, void* partial, size_t partial_bytes,
+                      void* cbuf) {
     using C = cx_t<T>;
     const int nch = c.nch();
     if (nch == 0) return hipSuccess;
+    if (nseg < 1 || nseg > kMaxSeg) return hipErrorInvalidValue;
     const bool f64 = sizeof(T) == 8;
-    const C* P = static_cast<const C*>(f64 ? k.P64.ptr : k.P32.ptr);
-    const int nsplit = choose_split(nch, nb, k.npw);
-    if ((size_t)nsplit * nch * nb * sizeof(C) > partial_bytes) return hipErrorInvalidValue;
+    const C* P[kMaxSeg];
+    for (int i = 0; i < nseg; ++i) P[i] = static_cast<const C*>(f64 ? segs[i].k->P64.ptr : segs[i].k->P32.ptr);
+    const int nsplit = choose_split(nch, nb, min_npw(segs, nseg), nseg);
+    if ((size_t)nsplit * nch * nb * sizeof(C) * nseg > partial_bytes) return hipErrorInvalidValue;
+    C* part = static_cast<C*>(partial);
+    C* cb = static_cast<C*>(cbuf);
     timer_begin(c);
-    hipError_t e = gemm_proj<T>(c, P, nch, k.npw, static_cast<const C*>(psi), ld, nb,
-                                static_cast<C*>(partial), nch, 0, nsplit);
+    hipError_t e = gemm_proj<T>(c, segs, nseg, P, nch, nb, part, nch, 0, nsplit);
     timer_end(c, "proj_gemm");
     if (e != hipSuccess) return e;
-    int64_t tot = (int64_t)nch * nb;
     timer_begin(c);
-    hipLaunchKernelGGL((k_reduce_scale<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
-                       c.stream, static_cast<const C*>(partial), nsplit, nch, nb,
-                       static_cast<const double*>(c.lam64.ptr), nch, static_cast<C*>(cbuf));
+    e = reduce_scale<T>(c, nseg, part, nsplit, nch, nb, cb);
     timer_end(c, "proj_reduce");
+    if (e != hipSuccess) return e;
+    ApplySegK<T> ap[kMaxSeg];
+    for (int i = 0; i < nseg; ++i)
+        ap[i] = ApplySegK<T>{P[i], cb + (size_t)i * nch * nb, static_cast<C*>(segs[i].out[0]), nullptr, nullptr,
+                             (int64_t)segs[i].k->npw, (int64_t)nb, segs[i].ldo, 0, segs[i].k->npw, 0};
     timer_begin(c);
-    e = gemm_apply<T>(c, P, k.npw, static_cast<const C*>(cbuf), nb, nch, static_cast<C*>(hpsi),
-                      ldo, true, nullptr, nullptr, 0);
+    e = gemm_apply<T>(c, nseg, ap, nb, nch, true);
     timer_end(c, "proj_apply");
     return e;
 }
 
 template <typename T>
-hipError_t launch_dhdk(Ctx& c, const KptDev& k, int nb, const void* psi, int64_t ld, void* ox,
-                       void* oy, void* oz, int64_t ldo, bool accumulate, void* partial,
+hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accumulate, void* partial,
                        size_t partial_bytes, void* cbuf) {
     using C = cx_t<T>;
     const int nch = c.nch();
     const bool f64 = sizeof(T) == 8;
-    void* outs[3] = {ox, oy, oz};
-    const double* Kd = static_cast<const double*>(k.K.ptr);
+    if (nseg < 1 || nseg > kMaxSeg) return hipErrorInvalidValue;
     hipError_t e = hipSuccess;
+    ApplySegK<T> ap[kMaxSeg];
     if (nch == 0) {
         // kinetic part only: out_a (+)= K_a psi  (gradient.cpp:139-141)
-        for (int a = 0; a < 3; ++a) {
-            if (!outs[a]) continue;
-            e = gemm_apply<T>(c, nullptr, k.npw, nullptr, nb, 0, static_cast<C*>(outs[a]), ldo,
-                              accumulate, Kd + (size_t)a * k.npw, static_cast<const C*>(psi), ld);
-            if (e != hipSuccess) return e;
+        for (int a = 0; a < 3 && e == hipSuccess; ++a) {
+            for (int i = 0; i < nseg; ++i) {
+                const KptDev& k = *segs[i].k;
+                ap[i] = ApplySegK<T>{nullptr, nullptr, static_cast<C*>(segs[i].out[a]),
+                                     static_cast<const double*>(k.K.ptr) + (size_t)a * k.npw,
+                                     static_cast<const C*>(segs[i].psi), 0, 0, segs[i].ldo, segs[i].ld, k.npw, 0};
+            }
+            e = gemm_apply<T>(c, nseg, ap, nb, 0, accumulate);
         }
-        return hipSuccess;
+        return e;
     }
-    const C* P = static_cast<const C*>(f64 ? k.P64.ptr : k.P32.ptr);
-    const C* dP = static_cast<const C*>(f64 ? k.dP64.ptr : k.dP32.ptr);
+    const C* P[kMaxSeg];
+    const C* dP[kMaxSeg];
+    for (int i = 0; i < nseg; ++i) {
+        P[i] = static_cast<const C*>(f64 ? segs[i].k->P64.ptr : segs[i].k->P32.ptr);
+        dP[i] = static_cast<const C*>(f64 ? segs[i].k->dP64.ptr : segs[i].k->dP32.ptr);
+    }
     const int Mtot = 4 * nch;
-    const int nsplit = choose_split(Mtot, nb, k.npw);
-    if ((size_t)nsplit * Mtot * nb * sizeof(C) > partial_bytes) return hipErrorInvalidValue;
+    const int nsplit = choose_split(Mtot, nb, min_npw(segs, nseg), nseg);
+    if ((size_t)nsplit * Mtot * nb * sizeof(C) * nseg > partial_bytes) return hipErrorInvalidValue;
+    C* part = static_cast<C*>(partial);
+    C* cb = static_cast<C*>(cbuf);
     // <p|psi> and <dp_a|psi> for a = x, y, z in one partial buffer
     timer_begin(c);
-    e = gemm_proj<T>(c, P, nch, k.npw, static_cast<const C*>(psi), ld, nb,
-                     static_cast<C*>(partial), Mtot, 0, nsplit);
-    if (e == hipSuccess)
-        e = gemm_proj<T>(c, dP, 3 * nch, k.npw, static_cast<const C*>(psi), ld, nb,
-                         static_cast<C*>(partial), Mtot, nch, nsplit);
+    e = gemm_proj<T>(c, segs, nseg, P, nch, nb, part, Mtot, 0, nsplit);
+    if (e == hipSuccess) e = gemm_proj<T>(c, segs, nseg, dP, 3 * nch, nb, part, Mtot, nch, nsplit);
     timer_end(c, "proj_gemm");
     if (e != hipSuccess) return e;
-    int64_t tot = (int64_t)Mtot * nb;
     timer_begin(c);
-    hipLaunchKernelGGL((k_reduce_scale<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
-                       c.stream, static_cast<const C*>(partial), nsplit, Mtot, nb,
-                       static_cast<const double*>(c.lam64.ptr), nch, static_cast<C*>(cbuf));
+    e = reduce_scale<T>(c, nseg, part, nsplit, Mtot, nb, cb);
     timer_end(c, "proj_reduce");
-    const C* cb = static_cast<const C*>(cbuf);
+    if (e != hipSuccess) return e;
     timer_begin(c);
     for (int a = 0; a < 3 && e == hipSuccess; ++a) {
-        if (!outs[a]) continue;
-        C* o = static_cast<C*>(outs[a]);
         // out_a (+)= K_a psi + sum_c p_c lambda_c <dp_a,c|psi>
-        e = gemm_apply<T>(c, P, k.npw, cb + (size_t)(1 + a) * nch * nb, nb, nch, o, ldo,
-                          accumulate, Kd + (size_t)a * k.npw, static_cast<const C*>(psi), ld);
+        for (int i = 0; i < nseg; ++i) {
+            const KptDev& k = *segs[i].k;
+            const C* cbs = cb + (size_t)i * Mtot * nb;
+            ap[i] = ApplySegK<T>{P[i], cbs + (size_t)(1 + a) * nch * nb, static_cast<C*>(segs[i].out[a]),
+                                 static_cast<const double*>(k.K.ptr) + (size_t)a * k.npw,
+                                 static_cast<const C*>(segs[i].psi), (int64_t)k.npw, (int64_t)nb, segs[i].ldo,
+                                 segs[i].ld, k.npw, 0};
+        }
+        e = gemm_apply<T>(c, nseg, ap, nb, nch, accumulate);
+        if (e != hipSuccess) break;
         // out_a += sum_c dp_a,c lambda_c <p_c|psi>
-        if (e == hipSuccess)
-            e = gemm_apply<T>(c, dP + (size_t)a * nch * k.npw, k.npw, cb, nb, nch, o, ldo, true,
-                              nullptr, nullptr, 0);
+        for (int i = 0; i < nseg; ++i) {
+            const KptDev& k = *segs[i].k;
+            ap[i] = ApplySegK<T>{dP[i] + (size_t)a * nch * k.npw, cb + (size_t)i * Mtot * nb,
+                                 static_cast<C*>(segs[i].out[a]), nullptr, nullptr, (int64_t)k.npw, (int64_t)nb,
+                                 segs[i].ldo, 0, k.npw, 0};
+        }
+        e = gemm_apply<T>(c, nseg, ap, nb, nch, true);
     }
     timer_end(c, "proj_apply");
     return e;
 }
 
-template hipError_t launch_vnl<double>(Ctx&, const KptDev&, int, const void*, int64_t, void*,
-                                       int64_t, void*, size_t, void*);
-template hipError_t launch_vnl<float>(Ctx&, const KptDev&, int, const void*, int64_t, void*,
-                                      int64_t, void*, size_t, void*);
-template hipError_t launch_dhdk<double>(Ctx&, const KptDev&, int, const void*, int64_t, void*,
-                                        void*, void*, int64_t, bool, void*, size_t, void*);
-template hipError_t launch_dhdk<float>(Ctx&, const KptDev&, int, const void*, int64_t, void*,
-                                       void*, void*, int64_t, bool, void*, size_t, void*);
+template hipError_t launch_vnl<double>(Ctx&, const ApplySeg*, int, int, void*, size_t, void*);
+template hipError_t launch_vnl<float>(Ctx&, const ApplySeg*, int, int, void*, size_t, void*);
+template hipError_t launch_dhdk<double>(Ctx&, const ApplySeg*, int, int, bool, void*, size_t, void*);
+template hipError_t launch_dhdk<float>(Ctx&, const ApplySeg*, int, int, bool, void*, size_t, void*);
 
 // ---------------------------------------------------------------------------
 //  FP64 Rayleigh quotients

@@ -186,29 +186,57 @@ class Hamiltonian:
                    self._ctx, "effm_rayleigh")
         return E
 
-    def apply_multi(self, ikpts, psis, hpsis, prec: str = "fp64", overwrite: bool = False):
-        """H|psi> for several k-points (ragged npw) in one call; device tensors."""
-        p = PREC[prec]
+    def _batch(self, ikpts, psis, outs, p):
+        """ctypes arrays of a multi-k batch: every tensor on the device, one
+        band count, outputs with the leading dimension of their psi."""
         nk = len(ikpts)
         ks = (C.c_int * nk)(*ikpts)
         pp = (C.c_void_p * nk)()
-        hp = (C.c_void_p * nk)()
         lds = (C.c_int64 * nk)()
+        ops = [None if o is None else (C.c_void_p * nk)() for o in outs]
         nb = None
         for i in range(nk):
             a, nbi, ld, host = _as_2d_info(psis[i], p)
-            b, nbo, ldo, host_o = _as_2d_info(hpsis[i], p)
-            if host or host_o or ld != ldo:
-                raise ValueError("apply_multi takes device tensors with equal leading dimensions")
+            if host:
+                raise ValueError("batched applies take device tensors")
             if nb is not None and nbi != nb:
                 raise ValueError("every k-point needs the same band count")
             nb = nbi
-            pp[i], hp[i], lds[i] = a, b, ld
+            pp[i], lds[i] = a, ld
+            for o, arr in zip(outs, ops):
+                if o is None:
+                    continue
+                b, nbo, ldo, host_o = _as_2d_info(o[i], p)
+                if host_o or ldo != ld or nbo < nbi:
+                    raise ValueError("outputs must be device tensors shaped like their psi")
+                arr[i] = b
+        return nk, ks, nb or 0, pp, lds, ops
+
+    def apply_multi(self, ikpts, psis, hpsis, prec: str = "fp64", overwrite: bool = False,
+                    graph: bool = False):
+        """H|psi> for several k-points (ragged npw) in one batched call;
+        device tensors [nb, >= npw(k)].  graph=True replays a captured
+        hipGraph of the launches (same pointers and shapes as a previous
+        call: the contents of psi may change, not the buffers)."""
+        p = PREC[prec]
+        nk, ks, nb, pp, lds, (hp,) = self._batch(ikpts, psis, [hpsis], p)
         self._sync_stream()
-        flags = _lib.EFFM_F_OVERWRITE if overwrite else 0
-        _lib.check(self._lib.effm_hpsi_multi(self._ctx, nk, ks, nb or 0, pp, lds, hp, p, flags),
+        flags = (_lib.EFFM_F_OVERWRITE if overwrite else 0) | (_lib.EFFM_F_GRAPH if graph else 0)
+        _lib.check(self._lib.effm_hpsi_multi(self._ctx, nk, ks, nb, pp, lds, hp, p, flags),
                    self._ctx, "effm_hpsi_multi")
         return hpsis
+
+    def gradient_multi(self, ikpts, psis, outs: "GradHPsi", prec: str = "fp64", overwrite: bool = False,
+                       graph: bool = False):
+        """dH/dk|psi> for several k-points in one batched call; outs.x / .y /
+        .z are per-k lists of device tensors (None skips a direction)."""
+        p = PREC[prec]
+        nk, ks, nb, pp, lds, (ox, oy, oz) = self._batch(ikpts, psis, [outs.x, outs.y, outs.z], p)
+        self._sync_stream()
+        flags = (_lib.EFFM_F_OVERWRITE if overwrite else 0) | (_lib.EFFM_F_GRAPH if graph else 0)
+        _lib.check(self._lib.effm_dhdk_multi(self._ctx, nk, ks, nb, pp, lds, ox, oy, oz, p, flags),
+                   self._ctx, "effm_dhdk_multi")
+        return outs
 
     def set_timing(self, on: bool):
         self._lib.effm_set_timing(self._ctx, 1 if on else 0)
@@ -239,7 +267,7 @@ class Callable:
         self.dim()
 
     def __call__(self, psi, hpsi, n_bands: Optional[int] = None, prec: str = "fp64",
-                 overwrite: bool = False):
+                 overwrite: bool = False, graph: bool = False):
         p = PREC[prec]
         pp, nb, ld, host = _as_2d_info(psi, p)
         hp, nbo, ldo, host_o = _as_2d_info(hpsi, p)
@@ -247,7 +275,8 @@ class Callable:
             raise TypeError("psi and hpsi must both be host or both be device arrays")
         if n_bands is not None:
             nb = n_bands
-        flags = (_lib.EFFM_F_OVERWRITE if overwrite else 0) | (_lib.EFFM_F_HOST if host else 0)
+        flags = ((_lib.EFFM_F_OVERWRITE if overwrite else 0) | (_lib.EFFM_F_HOST if host else 0)
+                 | (_lib.EFFM_F_GRAPH if graph else 0))
         if not host:
             self._h._sync_stream()
         _lib.check(self._h._lib.effm_hpsi(self._h._ctx, self._ikpt, nb, pp, ld, hp, ldo, p, flags),

@@ -47,6 +47,10 @@ extern "C" {
 /* ---- flags -------------------------------------------------------------- */
 #define EFFM_F_OVERWRITE  0x1  /* out = H psi instead of out += H psi */
 #define EFFM_F_HOST       0x2  /* psi / outputs are host pointers */
+#define EFFM_F_GRAPH      0x4  /* replay a hipGraph of this apply: captured on
+                                  first use, keyed by k-points, pointers,
+                                  leading dimensions, nb, precision, flags
+                                  and stream; device pointers only */
 
 typedef struct effm_ctx effm_ctx;
 
@@ -126,13 +130,24 @@ int effm_dhdk(effm_ctx* ctx, int ikpt, int nb, const void* psi, int64_t ld,
               void* out_x, void* out_y, void* out_z, int64_t ld_out,
               int prec, int flags);
 
-/* Batched ragged apply: nk k-points in one set of launches, band-major
- * psi/hpsi blocks of nb bands per k-point at psi_ptrs[i] / hpsi_ptrs[i]
- * (device pointers), ld[i] >= npw(k).  No reference counterpart (the
- * reference applies one band at one k); results equal effm_hpsi per k. */
+/* Batched ragged apply: nk k-points, each with its own npw, in one set of
+ * launches (up to 16 k-points per launch; every kernel walks the
+ * concatenated (k-point, plane / tile, band) work of the batch).  Band-major
+ * psi / hpsi blocks of nb bands per k-point at psi_ptrs[i] / hpsi_ptrs[i]
+ * (device pointers), ld[i] >= npw(k) for both.  No reference counterpart
+ * (the reference applies one band at one k); results equal effm_hpsi per
+ * k-point. */
 int effm_hpsi_multi(effm_ctx* ctx, int nk, const int* ikpts, int nb,
                     const void* const* psi_ptrs, const int64_t* ld,
                     void* const* hpsi_ptrs, int prec, int flags);
+
+/* Batched ragged dH/dk: as effm_hpsi_multi, outputs out_x[i] / out_y[i] /
+ * out_z[i] (a NULL array skips that direction for every k-point).  No
+ * reference counterpart; results equal effm_dhdk per k-point. */
+int effm_dhdk_multi(effm_ctx* ctx, int nk, const int* ikpts, int nb,
+                    const void* const* psi_ptrs, const int64_t* ld,
+                    void* const* out_x, void* const* out_y, void* const* out_z,
+                    int prec, int flags);
 
 /* FP64 Rayleigh quotients E_b = Re<psi_b|hpsi_b> / <psi_b|psi_b> over npw
  * elements (test_hpsi_eigen.cpp:49-56 computes this on the host); E is a

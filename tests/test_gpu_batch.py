@@ -1,0 +1,187 @@
+"""Batched ragged applies (effm_hpsi_multi / effm_dhdk_multi) and hipGraph
+replay (EFFM_F_GRAPH), through the C-ABI.
+
+Tolerances: the FP64 batch equals the per-k FP64 apply to 1e-13 relative
+per band (the split-K partial sums may be grouped differently) and the
+oracle to 1e-12; a graph replay equals the eager apply bit for bit (same
+kernels, same arguments)."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import workload
+from effmass_amd import synth
+from effmass_amd.hamiltonian import GradHPsi, Hamiltonian
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def band_rel(got, ref):
+    num = np.linalg.norm(got - ref, axis=-1)
+    den = np.linalg.norm(ref, axis=-1)
+    return float((num / den).max())
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _small_multi(nk, grid=(36, 36, 36), nb=6, seed=11):
+    """fcc Si2 at nk random k-points: ragged npw, quick for the oracle."""
+    rng = np.random.default_rng(seed)
+    ks = [rng.uniform(-0.5, 0.5, 3) for _ in range(nk)]
+    return synth.custom(*synth.fcc_primitive(10.26), grid, 30.0, ks, nb, seed=seed)
+
+
+def _load(wl):
+    H = Hamiltonian(wl.cell)
+    for ik, kp in enumerate(wl.kpts):
+        H.load_kpoint(ik, kp)
+    return H
+
+
+def test_multi_config2_shapes_equals_single_k():
+    """64-atom cell, 5 k-points of the 4x4x2 grid (ragged npw), 8 bands: the
+    batched launch against one effm_hpsi per k-point and the oracle."""
+    torch = _torch()
+    wl = workload(2, nb=8, nk=5)
+    assert len({kp.npw for kp in wl.kpts}) > 1
+    log(f"config-2 shapes npw {[kp.npw for kp in wl.kpts]}")
+    H = _load(wl)
+    psis = [torch.from_numpy(p).cuda() for p in wl.psi]
+    outs = [torch.zeros_like(p) for p in psis]
+    H.apply_multi(list(range(5)), psis, outs, overwrite=True)
+    for ik in range(5):
+        single = torch.zeros_like(psis[ik])
+        H.at_k(ik)(psis[ik], single, overwrite=True)
+        torch.cuda.synchronize()
+        e = band_rel(outs[ik].cpu().numpy(), single.cpu().numpy())
+        log(f"k {ik}: batch vs single {e:.2e}")
+        assert e <= 1e-13
+    ref = oracle.hpsi(wl.cell, wl.kpts[3], wl.psi[3], nthreads=8)
+    assert band_rel(outs[3].cpu().numpy(), ref) <= 1e-12
+    # mixed: batch vs single within FP32 rounding
+    p32 = [p.to(torch.complex64) for p in psis]
+    o32 = [torch.zeros_like(p) for p in p32]
+    H.apply_multi(list(range(5)), p32, o32, prec="mixed", overwrite=True)
+    s32 = torch.zeros_like(p32[1])
+    H.at_k(1)(p32[1], s32, prec="mixed", overwrite=True)
+    torch.cuda.synchronize()
+    assert band_rel(o32[1].cpu().numpy().astype(np.complex128), s32.cpu().numpy().astype(np.complex128)) <= 1e-5
+    H.close()
+
+
+def test_multi_over_sixteen_kpoints_and_accumulate():
+    """18 k-points (two launches of at most 16), accumulating onto a start
+    value, every k-point against the oracle."""
+    torch = _torch()
+    wl = _small_multi(18)
+    H = _load(wl)
+    psis = [torch.from_numpy(p).cuda() for p in wl.psi]
+    base = [torch.from_numpy(np.full(p.shape, 0.25 - 0.5j)).cuda() for p in wl.psi]
+    outs = [b.clone() for b in base]
+    H.apply_multi(list(range(18)), psis, outs)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for ik in range(18):
+        ref = oracle.hpsi(wl.cell, wl.kpts[ik], wl.psi[ik]) + base[ik].cpu().numpy()
+        worst = max(worst, band_rel(outs[ik].cpu().numpy(), ref))
+    log(f"18 k-points worst rel err {worst:.2e}")
+    assert worst <= 1e-12
+    H.close()
+
+
+def test_multi_generic_grid_kernels():
+    """A grid without compile-time plans (28 x 30 x 32) takes the generic
+    batched kernels (one grid slice per segment)."""
+    torch = _torch()
+    ks = [np.array([0.13, 0.0, -0.27]), np.array([0.5, 0.25, 0.0]), np.zeros(3)]
+    wl = synth.custom(*synth.fcc_primitive(10.26), (28, 30, 32), 12.0, ks, 4, seed=5)
+    H = _load(wl)
+    psis = [torch.from_numpy(p).cuda() for p in wl.psi]
+    outs = [torch.zeros_like(p) for p in psis]
+    H.apply_multi([0, 1, 2], psis, outs, overwrite=True)
+    torch.cuda.synchronize()
+    for ik in range(3):
+        assert band_rel(outs[ik].cpu().numpy(), oracle.hpsi(wl.cell, wl.kpts[ik], wl.psi[ik])) <= 1e-12
+    H.close()
+
+
+def test_dhdk_multi_equals_single_k():
+    torch = _torch()
+    wl = _small_multi(4)
+    H = _load(wl)
+    psis = [torch.from_numpy(p).cuda() for p in wl.psi]
+    ox = [torch.zeros_like(p) for p in psis]
+    oz = [torch.zeros_like(p) for p in psis]
+    H.gradient_multi([0, 1, 2, 3], psis, GradHPsi(ox, None, oz), overwrite=True)
+    for ik in range(4):
+        sx, sz = torch.zeros_like(psis[ik]), torch.zeros_like(psis[ik])
+        H.gradient().at_k(ik)(psis[ik], GradHPsi(sx, None, sz), overwrite=True)
+        torch.cuda.synchronize()
+        assert band_rel(ox[ik].cpu().numpy(), sx.cpu().numpy()) <= 1e-13
+        assert band_rel(oz[ik].cpu().numpy(), sz.cpu().numpy()) <= 1e-13
+    gx, gy, gz = oracle.dhdk(wl.cell, wl.kpts[2], wl.psi[2])
+    assert band_rel(ox[2].cpu().numpy(), gx) <= 1e-12
+    assert band_rel(oz[2].cpu().numpy(), gz) <= 1e-12
+    H.close()
+
+
+def test_graph_replay_matches_eager():
+    """Captured apply sequences replayed several times, with eager work and
+    new psi contents in between; every replay equals the eager result."""
+    torch = _torch()
+    wl = _small_multi(3, nb=5, seed=21)
+    H = _load(wl)
+    psis = [torch.from_numpy(p).cuda() for p in wl.psi]
+    outs = [torch.zeros_like(p) for p in psis]
+    eager = [torch.zeros_like(p) for p in psis]
+    g = [torch.zeros_like(p) for p in psis]
+    rng = np.random.default_rng(3)
+    for rep in range(4):
+        for p in psis:               # same buffers, new contents
+            p.copy_(torch.from_numpy(rng.standard_normal(p.shape) + 1j * rng.standard_normal(p.shape)))
+        H.apply_multi([0, 1, 2], psis, g, overwrite=True, graph=True)
+        H.apply_multi([0, 1, 2], psis, eager, overwrite=True)
+        torch.cuda.synchronize()
+        for a, b in zip(g, eager):
+            assert torch.equal(a, b), f"replay {rep} differs"
+        # single-k H|psi> and dH/dk replays
+        h1 = torch.zeros_like(psis[1])
+        H.at_k(1)(psis[1], h1, overwrite=True, graph=True)
+        h2 = torch.zeros_like(psis[1])
+        H.at_k(1)(psis[1], h2, overwrite=True)
+        torch.cuda.synchronize()
+        assert torch.equal(h1, h2)
+        dg = [torch.zeros_like(p) for p in psis]
+        de = [torch.zeros_like(p) for p in psis]
+        H.gradient_multi([0, 1, 2], psis, GradHPsi(None, dg, None), overwrite=True, graph=True)
+        H.gradient_multi([0, 1, 2], psis, GradHPsi(None, de, None), overwrite=True)
+        torch.cuda.synchronize()
+        for a, b in zip(dg, de):
+            assert torch.equal(a, b), f"dH/dk replay {rep} differs"
+    # accumulate mode through a graph: out += H psi on every replay
+    acc = torch.zeros_like(psis[0])
+    for rep in range(3):
+        H.at_k(0)(psis[0], acc, graph=True)
+    one = torch.zeros_like(psis[0])
+    H.at_k(0)(psis[0], one, overwrite=True)
+    torch.cuda.synchronize()
+    assert band_rel(acc.cpu().numpy(), 3 * one.cpu().numpy()) <= 1e-14
+    # a new allocation (another k-point) drops the cached graphs; the next
+    # graph call captures again and is still right
+    H.load_kpoint(3, wl.kpts[0])
+    H.apply_multi([0, 1, 2], psis, g, overwrite=True, graph=True)
+    torch.cuda.synchronize()
+    for ik in range(3):
+        assert band_rel(g[ik].cpu().numpy(), oracle.hpsi(wl.cell, wl.kpts[ik], psis[ik].cpu().numpy())) <= 1e-12
+    H.close()

@@ -205,12 +205,29 @@ def test_error_paths():
     far.kin = 0.5 * (far.K ** 2).sum(1)
     with pytest.raises(_lib.EffmDomainError):
         H.load_kpoint(0, far)
+    H.load_kpoint(0, kp)
+    resident, _ = H.mem_info()                              # cell + k-point data
     H.close()
-    with pytest.raises(_lib.EffmOutOfMemory):                # tiny HBM budget
-        Hs = Hamiltonian(wl.cell, hbm_budget=2 << 20)
+    proj = wl.cell.nchannels() * kp.npw * 16                # FP64 projector matrix
+    plane_bytes = 36 * 36 * 16
+    p = torch.from_numpy(wl.psi[0]).cuda()
+    # the FFT workspace is chunked to fit the budget: room for about two
+    # bands' planes still gives the right answer ...
+    nplanes = len(np.unique(kp.gidx[:, 0] % 36))
+    Hs = Hamiltonian(wl.cell, hbm_budget=resident + proj + 2 * nplanes * plane_bytes + (1 << 20))
+    try:
+        Hs.load_kpoint(0, kp)
+        h = torch.zeros_like(p)
+        Hs.at_k(0)(p, h)
+        torch.cuda.synchronize()
+        assert band_rel(h.cpu().numpy(), oracle.hpsi(wl.cell, kp, wl.psi[0])) <= 1e-12
+    finally:
+        Hs.close()
+    # ... and no room for one band's planes fails loudly
+    with pytest.raises(_lib.EffmOutOfMemory):
+        Hs = Hamiltonian(wl.cell, hbm_budget=resident + proj + nplanes * plane_bytes // 4)
         try:
             Hs.load_kpoint(0, kp)
-            p = torch.from_numpy(wl.psi[0]).cuda()
             Hs.at_k(0)(p, torch.zeros_like(p))
         finally:
             Hs.close()
