@@ -102,6 +102,8 @@ def main():
             op(p, h, prec=prec, overwrite=True)
 
     precs = (args.only,) if args.only else ("fp64", "mixed")
+    # launch shapes picked by the autotuner on this cell, outside the timed region
+    tuned = {prec: dict(zip(("pencil_bands", "gemm_wgs"), H.autotune(0, nb, prec))) for prec in precs}
     res = {}
     for prec in precs:
         run(prec, args.warmup)
@@ -228,6 +230,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "tuned_launch_shapes": tuned,
         "config2": config2,
         "serving": serving_stats,
         "setup_s": setup_s,

@@ -266,7 +266,7 @@ static int run_hpsi(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool acc, bo
         if (c.nch() > 0) {
             int mn = sg[0].k->npw;
             for (int i = 1; i < ng; ++i) mn = std::min(mn, sg[i].k->npw);
-            if ((rc = ensure(c, c.partial, vnl_partial_bytes(c, c.nch(), nb, mn, ng, sizeof(C)),
+            if ((rc = ensure(c, c.partial, vnl_partial_bytes<T>(c, c.nch(), nb, mn, ng),
                              "projection partial sums")))
                 return rc;
             if ((rc = ensure(c, c.cbuf, (size_t)ng * c.nch() * nb * sizeof(C), "projections"))) return rc;
@@ -294,7 +294,7 @@ static int run_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool acc, bo
         if (c.nch() > 0) {
             int mn = sg[0].k->npw;
             for (int i = 1; i < ng; ++i) mn = std::min(mn, sg[i].k->npw);
-            if ((rc = ensure(c, c.partial, vnl_partial_bytes(c, 4 * c.nch(), nb, mn, ng, sizeof(C)),
+            if ((rc = ensure(c, c.partial, vnl_partial_bytes<T>(c, 4 * c.nch(), nb, mn, ng),
                              "projection partial sums")))
                 return rc;
             if ((rc = ensure(c, c.cbuf, (size_t)ng * 4 * c.nch() * nb * sizeof(C), "projections"))) return rc;
@@ -373,6 +373,72 @@ static int run_maybe_graph(Ctx& c, bool graph, std::vector<int64_t> key, F&& enq
     c.graphs.push_back({std::move(key), exec, ++c.graph_clock});
     HIP_TRY(hipGraphLaunch(exec, c.stream), "graph launch");
     return kOk;
+}
+
+// ---------------------------------------------------------------------------
+//  autotuner: time each candidate launch shape on this cell / k-point
+// ---------------------------------------------------------------------------
+template <typename T>
+static int autotune(Ctx& c, KptDev& k, int nb, int* pencil_bands, int* gemm_wgs) {
+    using C = cx_t<T>;
+    const size_t bytes = (size_t)nb * k.npw * sizeof(C);
+    int rc;
+    if ((rc = ensure(c, c.stage_in, bytes, "tuning psi")) || (rc = ensure(c, c.stage_out0, bytes, "tuning hpsi")))
+        return rc;
+    // any non-zero, non-denormal pattern: timings do not depend on the values
+    HIP_TRY(hipMemsetAsync(c.stage_in.ptr, 0x3c, bytes, c.stream), "tuning fill");
+    ApplySeg seg{&k, c.stage_in.ptr, k.npw, {c.stage_out0.ptr, nullptr, nullptr}, k.npw};
+    Ctx::Tune& tn = c.tune[sizeof(T) == 8 ? 0 : 1];
+    const bool timing = c.timing;
+    c.timing = false;
+    hipEvent_t ev[2];
+    HIP_TRY(hipEventCreate(&ev[0]), "tuning event");
+    if (hipEventCreate(&ev[1]) != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        return fail(c, kErrHip, "tuning event");
+    }
+    // best of three applies after one warm-up, for the current shape
+    auto measure = [&](float* best) -> int {
+        *best = 1e30f;
+        int r = run_hpsi<T>(c, &seg, 1, nb, false, false);
+        if (r) return r;
+        for (int rep = 0; rep < 4; ++rep) {
+            HIP_TRY(hipEventRecord(ev[0], c.stream), "tuning record");
+            if ((r = run_hpsi<T>(c, &seg, 1, nb, false, true))) return r;
+            HIP_TRY(hipEventRecord(ev[1], c.stream), "tuning record");
+            HIP_TRY(hipEventSynchronize(ev[1]), "tuning sync");
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]), "tuning time");
+            if (rep > 0) *best = std::min(*best, ms);
+        }
+        return kOk;
+    };
+    rc = kOk;
+    float best = 1e30f, t = 0.f;
+    int pick = tn.pencil_bands;
+    for (int cand : kPencilBandChoices) {
+        tn.pencil_bands = cand;
+        if ((rc = measure(&t))) break;
+        if (t < best) best = t, pick = cand;
+    }
+    tn.pencil_bands = pick;
+    best = 1e30f;
+    pick = tn.gemm_wgs;
+    for (int cand : kGemmWgChoices) {
+        if (rc) break;
+        tn.gemm_wgs = cand;
+        if ((rc = measure(&t))) break;
+        if (t < best) best = t, pick = cand;
+    }
+    tn.gemm_wgs = pick;
+    tn.tuned = rc == kOk;
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    c.timing = timing;
+    ++c.epoch;    // captured graphs hold the old shapes: recapture
+    if (pencil_bands) *pencil_bands = tn.pencil_bands;
+    if (gemm_wgs) *gemm_wgs = tn.gemm_wgs;
+    return rc;
 }
 
 template <typename T>
@@ -828,6 +894,19 @@ int effm_dhdk_multi(effm_ctx* h, int nk, const int* ikpts, int nb, const void* c
     const bool graph = (flags & EFFM_F_GRAPH) != 0;
     return prec == EFFM_PREC_FP64 ? apply_dhdk<double>(c, segs.data(), nk, nb, acc, graph, prec)
                                   : apply_dhdk<float>(c, segs.data(), nk, nb, acc, graph, prec);
+}
+
+int effm_autotune(effm_ctx* h, int ikpt, int nb, int prec, int* pencil_bands, int* gemm_wgs) {
+    if (!h) return kErrInvalid;
+    Ctx& c = h->c;
+    (void)hipSetDevice(c.device);
+    int rc = check_kpt(c, ikpt);
+    if (rc) return rc;
+    if (nb <= 0) return fail(c, kErrInvalid, "nb must be positive");
+    if (prec != EFFM_PREC_FP64 && prec != EFFM_PREC_MIXED) return fail(c, kErrInvalid, "unknown precision");
+    KptDev& k = c.kpts[ikpt];
+    return prec == EFFM_PREC_FP64 ? autotune<double>(c, k, nb, pencil_bands, gemm_wgs)
+                                  : autotune<float>(c, k, nb, pencil_bands, gemm_wgs);
 }
 
 int effm_rayleigh(effm_ctx* h, int npw, int nb, const void* psi, int64_t ld, const void* hpsi,

@@ -115,6 +115,19 @@ struct Ctx {
     // per kernel: resident workgroups per CU (dynamic LDS attribute set once)
     std::map<const void*, int> occupancy;
 
+    // launch shapes, per precision (0 = FP64, 1 = mixed): bands per x-pencil
+    // workgroup and the workgroup count the split-K projection aims for.
+    // Defaults are the measured best on the 90^3 / 64-atom cell;
+    // effm_autotune re-picks them by timing the candidates on this cell.
+    struct Tune {
+        int pencil_bands = 8;
+        int gemm_wgs = 1024;
+        bool tuned = false;
+    };
+    Tune tune[2];
+    template <typename T>
+    const Tune& tuning() const { return tune[sizeof(T) == 8 ? 0 : 1]; }
+
     // captured apply sequences (EFFM_F_GRAPH), keyed by everything the
     // launches read; any device (re)allocation bumps epoch and drops them
     struct Graph {
@@ -219,7 +232,12 @@ hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accu
                        void* partial, size_t partial_bytes, void* cbuf);
 
 // split-K partial sums of nseg segments with nrows projector rows each
-size_t vnl_partial_bytes(const Ctx& c, int nrows, int nb, int min_npw, int nseg, size_t elem);
+template <typename T>
+size_t vnl_partial_bytes(const Ctx& c, int nrows, int nb, int min_npw, int nseg);
+
+// candidate launch shapes effm_autotune times
+constexpr int kPencilBandChoices[] = {4, 8, 16};
+constexpr int kGemmWgChoices[] = {512, 768, 1024, 1536, 2048};
 
 template <typename T>
 hipError_t launch_rayleigh(hipStream_t s, int npw, int nb, const void* psi, int64_t ld,

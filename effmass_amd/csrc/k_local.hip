@@ -522,7 +522,6 @@ k_xpencil_vloc_ct(const LocalBatch<T> bt, const T* __restrict__ vloc, int n2, in
 template <typename T, int N> struct PencilShape { static constexpr int TZ = 16; };
 template <> struct PencilShape<double, 90> { static constexpr int TZ = 18; };
 
-constexpr int kBandsPerPencilWG = 4;
 
 static int Plan2N(int n, int which) {
     switch (n) {
@@ -579,15 +578,24 @@ static hipError_t launch_planes_ct(Ctx& c, const LocalBatch<T>& bt, int nitems, 
     return hipGetLastError();
 }
 
-template <typename T, int N>
-static void launch_pencil_ct(Ctx& c, const LocalBatch<T>& bt, const T* vl, const cx_t<T>* tw1) {
+template <typename T, int N, int NBW>
+static void launch_pencil_ct_nbw(Ctx& c, const LocalBatch<T>& bt, const T* vl, const cx_t<T>* tw1) {
     constexpr int TZ = PencilShape<T, N>::TZ;
     constexpr int BLOCK = TZ * cmax(Plan2<N>::P1, Plan2<N>::P2);
     const int nzc = (c.n3 + TZ - 1) / TZ;
-    const int nbg = (bt.nb + kBandsPerPencilWG - 1) / kBandsPerPencilWG;
+    const int nbg = (bt.nb + NBW - 1) / NBW;
     dim3 gx(c.n2 * nzc, nbg * bt.nseg);
-    hipLaunchKernelGGL((k_xpencil_vloc_ct<T, N, TZ, kBandsPerPencilWG>), gx, dim3(BLOCK), 0, c.stream, bt,
-                       vl, c.n2, c.n3, tw1);
+    hipLaunchKernelGGL((k_xpencil_vloc_ct<T, N, TZ, NBW>), gx, dim3(BLOCK), 0, c.stream, bt, vl, c.n2, c.n3, tw1);
+}
+
+// bands per workgroup: the tuned value (one of kPencilBandChoices)
+template <typename T, int N>
+static void launch_pencil_ct(Ctx& c, const LocalBatch<T>& bt, const T* vl, const cx_t<T>* tw1) {
+    switch (c.tuning<T>().pencil_bands) {
+        case 4: launch_pencil_ct_nbw<T, N, 4>(c, bt, vl, tw1); break;
+        case 16: launch_pencil_ct_nbw<T, N, 16>(c, bt, vl, tw1); break;
+        default: launch_pencil_ct_nbw<T, N, 8>(c, bt, vl, tw1); break;
+    }
 }
 
 template <typename T>

@@ -536,21 +536,27 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
         }
 }
 
-// Split-K count: about 1024 workgroups over the batch, chunks of >= 256 G.
-static int choose_split(int M, int N, int K, int nseg) {
+// Split-K count: about `wgs` workgroups over the batch (the tuned target),
+// chunks of >= 256 G.
+static int choose_split(int wgs, int M, int N, int K, int nseg) {
     int tiles = ((M + 63) / 64) * ((N + 63) / 64) * std::max(1, nseg);
-    int want = (1024 + tiles - 1) / tiles;
+    int want = (wgs + tiles - 1) / tiles;
     int maxs = (K + 255) / 256;
     if (want > maxs) want = maxs;
     if (want < 1) want = 1;
     return want;
 }
 
-size_t vnl_partial_bytes(const Ctx& c, int nrows, int nb, int min_npw, int nseg, size_t elem) {
-    (void)c;
-    int ns = choose_split(nrows, nb, min_npw, nseg);
-    return (size_t)ns * nrows * nb * elem * nseg;
+// Partial-sum bytes for the largest split any candidate target can choose,
+// so that re-tuning never needs a reallocation.
+template <typename T>
+size_t vnl_partial_bytes(const Ctx& c, int nrows, int nb, int min_npw, int nseg) {
+    int ns = choose_split(c.tuning<T>().gemm_wgs, nrows, nb, min_npw, nseg);
+    for (int w : kGemmWgChoices) ns = std::max(ns, choose_split(w, nrows, nb, min_npw, nseg));
+    return (size_t)ns * nrows * nb * sizeof(cx_t<T>) * nseg;
 }
+template size_t vnl_partial_bytes<double>(const Ctx&, int, int, int, int);
+template size_t vnl_partial_bytes<float>(const Ctx&, int, int, int, int);
 
 static int min_npw(const ApplySeg* segs, int nseg) {
     int m = segs[0].k->npw;
@@ -621,7 +627,7 @@ hipError_t launch_vnl(Ctx& c, const ApplySeg* segs, int nseg, int nb, void* part
     const bool f64 = sizeof(T) == 8;
     const C* P[kMaxSeg];
     for (int i = 0; i < nseg; ++i) P[i] = static_cast<const C*>(f64 ? segs[i].k->P64.ptr : segs[i].k->P32.ptr);
-    const int nsplit = choose_split(nch, nb, min_npw(segs, nseg), nseg);
+    const int nsplit = choose_split(c.tuning<T>().gemm_wgs, nch, nb, min_npw(segs, nseg), nseg);
     if ((size_t)nsplit * nch * nb * sizeof(C) * nseg > partial_bytes) return hipErrorInvalidValue;
     C* part = static_cast<C*>(partial);
     C* cb = static_cast<C*>(cbuf);
@@ -672,7 +678,7 @@ hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accu
         dP[i] = static_cast<const C*>(f64 ? segs[i].k->dP64.ptr : segs[i].k->dP32.ptr);
     }
     const int Mtot = 4 * nch;
-    const int nsplit = choose_split(Mtot, nb, min_npw(segs, nseg), nseg);
+    const int nsplit = choose_split(c.tuning<T>().gemm_wgs, Mtot, nb, min_npw(segs, nseg), nseg);
     if ((size_t)nsplit * Mtot * nb * sizeof(C) * nseg > partial_bytes) return hipErrorInvalidValue;
     C* part = static_cast<C*>(partial);
     C* cb = static_cast<C*>(cbuf);

@@ -238,6 +238,16 @@ class Hamiltonian:
                    self._ctx, "effm_dhdk_multi")
         return outs
 
+    def autotune(self, ikpt: int, n_bands: int, prec: str = "fp64"):
+        """Time the candidate launch shapes at this k-point and keep the
+        fastest for `prec`; returns (bands per pencil workgroup, split-K
+        workgroup target)."""
+        pb, gw = C.c_int(), C.c_int()
+        self._sync_stream()
+        _lib.check(self._lib.effm_autotune(self._ctx, ikpt, n_bands, PREC[prec], C.byref(pb), C.byref(gw)),
+                   self._ctx, "effm_autotune")
+        return pb.value, gw.value
+
     def set_timing(self, on: bool):
         self._lib.effm_set_timing(self._ctx, 1 if on else 0)
 

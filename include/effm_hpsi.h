@@ -155,6 +155,14 @@ int effm_dhdk_multi(effm_ctx* ctx, int nk, const int* ikpts, int nb,
 int effm_rayleigh(effm_ctx* ctx, int npw, int nb, const void* psi, int64_t ld,
                   const void* hpsi, int64_t ld_h, int prec, double* E, int flags);
 
+/* Autotuner: times the candidate launch shapes (bands per x-pencil
+ * workgroup: 4 / 8 / 16; split-K workgroup target of the projection GEMM:
+ * 512 ... 2048) with nb bands at k-point ikpt in precision prec, on
+ * scratch buffers, and keeps the fastest for every later apply in that
+ * precision.  pencil_bands / gemm_wgs (may be NULL) receive the choice.
+ * No reference counterpart. */
+int effm_autotune(effm_ctx* ctx, int ikpt, int nb, int prec, int* pencil_bands, int* gemm_wgs);
+
 /* Per-kernel timing with HIP events on the context's stream (0 = off). */
 int effm_set_timing(effm_ctx* ctx, int enable);
 /* Accumulated milliseconds and launch count of a kernel class; names:

@@ -185,3 +185,23 @@ def test_graph_replay_matches_eager():
     for ik in range(3):
         assert band_rel(g[ik].cpu().numpy(), oracle.hpsi(wl.cell, wl.kpts[ik], psis[ik].cpu().numpy())) <= 1e-12
     H.close()
+
+
+def test_autotune_choices_keep_parity():
+    """The autotuner picks among the candidate shapes; every later apply,
+    whatever was picked, still matches the oracle."""
+    torch = _torch()
+    wl = workload(1, nb=24)
+    H = Hamiltonian(wl.cell)
+    H.load_kpoint(0, wl.kpts[0])
+    for prec in ("fp64", "mixed"):
+        pb, gw = H.autotune(0, 24, prec)
+        log(f"{prec}: pencil bands {pb}, split-K workgroups {gw}")
+        assert pb in (4, 8, 16) and gw in (512, 768, 1024, 1536, 2048)
+    psi = torch.from_numpy(wl.psi[0][:24]).cuda()
+    h = torch.zeros_like(psi)
+    H.at_k(0)(psi, h, overwrite=True)
+    torch.cuda.synchronize()
+    ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0][:4], nthreads=8)
+    assert band_rel(h[:4].cpu().numpy(), ref) <= 1e-12
+    H.close()
