@@ -286,9 +286,21 @@ def config2_leg(steps, warmup=1):
         t_graph = timed(step, steps)
         t_eager = timed(lambda: step(False), steps)
         t_loop = timed(per_k, steps)
+        # per-kernel split of one eager batched step (HIP events)
+        H.reset_timing()
+        H.set_timing(True)
+        step(False)
+        torch.cuda.synchronize()
+        split = {}
+        for name in ("plane_g2r", "xpencil_vloc", "plane_r2g", "proj_gemm", "proj_reduce", "proj_apply"):
+            ms, cnt = H.timing(name)
+            if cnt:
+                split[name] = {"ms": ms, "launches": cnt}
+        H.set_timing(False)
         applies = len(ks) * nb
         res[prec] = {"ms_per_step": t_graph * 1e3, "k_band_steps_per_s": applies / t_graph,
-                     "eager_batched_ms": t_eager * 1e3, "per_k_calls_ms": t_loop * 1e3}
+                     "eager_batched_ms": t_eager * 1e3, "per_k_calls_ms": t_loop * 1e3,
+                     "kernel_ms": split}
         del psis, hs, gr
         torch.cuda.empty_cache()
     H.close()

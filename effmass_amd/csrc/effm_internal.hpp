@@ -233,7 +233,7 @@ hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accu
 
 // split-K partial sums of nseg segments with nrows projector rows each
 template <typename T>
-size_t vnl_partial_bytes(const Ctx& c, int nrows, int nb, int min_npw, int nseg);
+size_t vnl_partial_bytes(Ctx& c, int nrows, int nb, int min_npw, int nseg);
 
 // candidate launch shapes effm_autotune times
 constexpr int kPencilBandChoices[] = {4, 8, 16};

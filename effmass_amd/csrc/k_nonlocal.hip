@@ -18,6 +18,7 @@
 // (src/math/sph_harmonics.cppm:120-335, 702-875) and separable structure
 // factor (src/H_psi/structure_factor.cppm:92-159).
 #include <algorithm>
+#include <cmath>
 
 #include "effm_internal.hpp"
 
@@ -411,6 +412,8 @@ struct ApplySegK {
     const cx_t<T>* A;      // [K][lda]
     const cx_t<T>* B;      // [K][ldb]
     cx_t<T>* out;          // [N][ldo]
+    const cx_t<T>* A2;     // optional second product A2 B2 (same K, lda, ldb), or NULL
+    const cx_t<T>* B2;
     const double* diag;    // [M] or NULL
     const cx_t<T>* src;    // [N][lds] (with diag)
     int64_t lda, ldb, ldo, lds;
@@ -424,7 +427,7 @@ struct ApplyBatch {
     ApplySegK<T> s[kMaxSeg];
 };
 
-template <typename T>
+template <typename T, bool TWO>
 __global__ void __launch_bounds__(256)
 k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     using C = cx_t<T>;
@@ -439,6 +442,10 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     const int L = L0 - sg.tile0;
     const C* __restrict__ A = sg.A;
     const C* __restrict__ B = sg.B;
+    const C* __restrict__ A2 = sg.A2;
+    const C* __restrict__ B2 = sg.B2;
+    // TWO: the second product continues the k loop, k in [K, 2K) reads A2 / B2
+    const int Ktot = TWO ? 2 * K : K;
     C* __restrict__ out = sg.out;
     const double* __restrict__ diag = sg.diag;
     const C* __restrict__ src = sg.src;
@@ -467,12 +474,16 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int k = kb + 4 * s + kk;
-            const bool kv = k < K;
-            const int kc = kv ? k : K - 1;
+            const bool kv = k < Ktot;
+            const int kt = kv ? k : Ktot - 1;
+            const bool sec = TWO && kt >= K;
+            const int kc = sec ? kt - K : kt;
+            const C* Ab = sec ? A2 : A;
+            const C* Bb = sec ? B2 : B;
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
-                const C x = A[(int64_t)kc * lda + mc[r]];
-                const C y = B[(int64_t)kc * ldb + nc[r]];
+                const C x = Ab[(int64_t)kc * lda + mc[r]];
+                const C y = Bb[(int64_t)kc * ldb + nc[r]];
                 a[buf][s][r] = kv ? x : C{0, 0};
                 b[buf][s][r] = y;
             }
@@ -502,15 +513,15 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
                 for (int q = 0; q < 2; ++q) k3[r][q] = mfma16(a[buf][s][r].y, sc[q], k3[r][q]);
         }
     };
-    if (K > 0) load(0, 0);
+    if (Ktot > 0) load(0, 0);
     int kb = 0;
-    for (; kb + 16 < K; kb += 32) {
+    for (; kb + 16 < Ktot; kb += 32) {
         load(1, kb + 16);
         mma(0);
-        if (kb + 32 < K) load(0, kb + 32);
+        if (kb + 32 < Ktot) load(0, kb + 32);
         mma(1);
     }
-    if (kb < K) mma(0);
+    if (kb < Ktot) mma(0);
     for (int r = 0; r < 2; ++r)
         for (int q = 0; q < 2; ++q) {
             const int n = n0 + q * 16 + i;
@@ -536,27 +547,50 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
         }
 }
 
-// Split-K count: about `wgs` workgroups over the batch (the tuned target),
-// chunks of >= 256 G.
-static int choose_split(int wgs, int M, int N, int K, int nseg) {
-    int tiles = ((M + 63) / 64) * ((N + 63) / 64) * std::max(1, nseg);
-    int want = (wgs + tiles - 1) / tiles;
-    int maxs = (K + 255) / 256;
-    if (want > maxs) want = maxs;
-    if (want < 1) want = 1;
-    return want;
+// Resident workgroups of the projection GEMM on the whole device.
+template <typename T>
+static int gemm_proj_slots(Ctx& c) {
+    const void* fn = reinterpret_cast<const void*>(&k_gemm_proj<T>);
+    auto it = c.occupancy.find(fn);
+    if (it == c.occupancy.end()) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        it = c.occupancy.emplace(fn, per_cu).first;
+    }
+    return c.ncu * it->second;
+}
+
+// Split-K count: at least `wgs` workgroups over the batch (the tuned
+// target), then, among up to three times that many splits, the one whose
+// last wave of workgroups is fullest (slots = resident workgroups); chunks
+// of >= 256 G.
+static int choose_split(int wgs, int slots, int M, int N, int K, int nseg) {
+    const int tiles = ((M + 63) / 64) * ((N + 63) / 64) * std::max(1, nseg);
+    const int maxs = std::max(1, (K + 255) / 256);
+    const int smin = std::min(maxs, std::max(1, (wgs + tiles - 1) / tiles));
+    const int smax = std::min(maxs, std::max(3, 3 * smin));
+    int best = smin;
+    double beff = -1.0;
+    for (int sp = smin; sp <= smax; ++sp) {
+        const double w = (double)tiles * sp;
+        const double eff = w / (std::ceil(w / slots) * slots);   // busy fraction of the waves
+        if (eff > beff + 1e-9) best = sp, beff = eff;
+    }
+    return best;
 }
 
 // Partial-sum bytes for the largest split any candidate target can choose,
 // so that re-tuning never needs a reallocation.
 template <typename T>
-size_t vnl_partial_bytes(const Ctx& c, int nrows, int nb, int min_npw, int nseg) {
-    int ns = choose_split(c.tuning<T>().gemm_wgs, nrows, nb, min_npw, nseg);
-    for (int w : kGemmWgChoices) ns = std::max(ns, choose_split(w, nrows, nb, min_npw, nseg));
+size_t vnl_partial_bytes(Ctx& c, int nrows, int nb, int min_npw, int nseg) {
+    const int slots = gemm_proj_slots<T>(c);
+    int ns = choose_split(c.tuning<T>().gemm_wgs, slots, nrows, nb, min_npw, nseg);
+    for (int w : kGemmWgChoices) ns = std::max(ns, choose_split(w, slots, nrows, nb, min_npw, nseg));
     return (size_t)ns * nrows * nb * sizeof(cx_t<T>) * nseg;
 }
-template size_t vnl_partial_bytes<double>(const Ctx&, int, int, int, int);
-template size_t vnl_partial_bytes<float>(const Ctx&, int, int, int, int);
+template size_t vnl_partial_bytes<double>(Ctx&, int, int, int, int);
+template size_t vnl_partial_bytes<float>(Ctx&, int, int, int, int);
 
 static int min_npw(const ApplySeg* segs, int nseg) {
     int m = segs[0].k->npw;
@@ -613,7 +647,11 @@ static hipError_t gemm_apply(Ctx& c, int nseg, const ApplySegK<T>* in, int nb, i
         tiles += ((s.M + 63) / 64) * ny;
     }
     if (bt.nseg == 0 || tiles == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_gemm_apply<T>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, acc ? 1 : 0);
+    // all segments of a call either have the second product or none
+    if (bt.s[0].A2)
+        hipLaunchKernelGGL((k_gemm_apply<T, true>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, acc ? 1 : 0);
+    else
+        hipLaunchKernelGGL((k_gemm_apply<T, false>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, acc ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -627,7 +665,7 @@ hipError_t launch_vnl(Ctx& c, const ApplySeg* segs, int nseg, int nb, void* part
     const bool f64 = sizeof(T) == 8;
     const C* P[kMaxSeg];
     for (int i = 0; i < nseg; ++i) P[i] = static_cast<const C*>(f64 ? segs[i].k->P64.ptr : segs[i].k->P32.ptr);
-    const int nsplit = choose_split(c.tuning<T>().gemm_wgs, nch, nb, min_npw(segs, nseg), nseg);
+    const int nsplit = choose_split(c.tuning<T>().gemm_wgs, gemm_proj_slots<T>(c), nch, nb, min_npw(segs, nseg), nseg);
     if ((size_t)nsplit * nch * nb * sizeof(C) * nseg > partial_bytes) return hipErrorInvalidValue;
     C* part = static_cast<C*>(partial);
     C* cb = static_cast<C*>(cbuf);
@@ -642,7 +680,8 @@ hipError_t launch_vnl(Ctx& c, const ApplySeg* segs, int nseg, int nb, void* part
     ApplySegK<T> ap[kMaxSeg];
     for (int i = 0; i < nseg; ++i)
         ap[i] = ApplySegK<T>{P[i], cb + (size_t)i * nch * nb, static_cast<C*>(segs[i].out[0]), nullptr, nullptr,
-                             (int64_t)segs[i].k->npw, (int64_t)nb, segs[i].ldo, 0, segs[i].k->npw, 0};
+                             nullptr, nullptr, (int64_t)segs[i].k->npw, (int64_t)nb, segs[i].ldo, 0,
+                             segs[i].k->npw, 0};
     timer_begin(c);
     e = gemm_apply<T>(c, nseg, ap, nb, nch, true);
     timer_end(c, "proj_apply");
@@ -663,7 +702,7 @@ hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accu
         for (int a = 0; a < 3 && e == hipSuccess; ++a) {
             for (int i = 0; i < nseg; ++i) {
                 const KptDev& k = *segs[i].k;
-                ap[i] = ApplySegK<T>{nullptr, nullptr, static_cast<C*>(segs[i].out[a]),
+                ap[i] = ApplySegK<T>{nullptr, nullptr, static_cast<C*>(segs[i].out[a]), nullptr, nullptr,
                                      static_cast<const double*>(k.K.ptr) + (size_t)a * k.npw,
                                      static_cast<const C*>(segs[i].psi), 0, 0, segs[i].ldo, segs[i].ld, k.npw, 0};
             }
@@ -678,7 +717,8 @@ hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accu
         dP[i] = static_cast<const C*>(f64 ? segs[i].k->dP64.ptr : segs[i].k->dP32.ptr);
     }
     const int Mtot = 4 * nch;
-    const int nsplit = choose_split(c.tuning<T>().gemm_wgs, Mtot, nb, min_npw(segs, nseg), nseg);
+    const int nsplit = choose_split(c.tuning<T>().gemm_wgs, gemm_proj_slots<T>(c), Mtot, nb, min_npw(segs, nseg),
+                                    nseg);
     if ((size_t)nsplit * Mtot * nb * sizeof(C) * nseg > partial_bytes) return hipErrorInvalidValue;
     C* part = static_cast<C*>(partial);
     C* cb = static_cast<C*>(cbuf);
@@ -694,25 +734,18 @@ hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accu
     if (e != hipSuccess) return e;
     timer_begin(c);
     for (int a = 0; a < 3 && e == hipSuccess; ++a) {
-        // out_a (+)= K_a psi + sum_c p_c lambda_c <dp_a,c|psi>
+        // out_a (+)= K_a psi + sum_c p_c lambda_c <dp_a,c|psi> + sum_c dp_a,c lambda_c <p_c|psi>:
+        // one launch, the two projector products concatenated along k
         for (int i = 0; i < nseg; ++i) {
             const KptDev& k = *segs[i].k;
             const C* cbs = cb + (size_t)i * Mtot * nb;
             ap[i] = ApplySegK<T>{P[i], cbs + (size_t)(1 + a) * nch * nb, static_cast<C*>(segs[i].out[a]),
+                                 dP[i] + (size_t)a * nch * k.npw, cbs,
                                  static_cast<const double*>(k.K.ptr) + (size_t)a * k.npw,
                                  static_cast<const C*>(segs[i].psi), (int64_t)k.npw, (int64_t)nb, segs[i].ldo,
                                  segs[i].ld, k.npw, 0};
         }
         e = gemm_apply<T>(c, nseg, ap, nb, nch, accumulate);
-        if (e != hipSuccess) break;
-        // out_a += sum_c dp_a,c lambda_c <p_c|psi>
-        for (int i = 0; i < nseg; ++i) {
-            const KptDev& k = *segs[i].k;
-            ap[i] = ApplySegK<T>{dP[i] + (size_t)a * nch * k.npw, cb + (size_t)i * Mtot * nb,
-                                 static_cast<C*>(segs[i].out[a]), nullptr, nullptr, (int64_t)k.npw, (int64_t)nb,
-                                 segs[i].ldo, 0, k.npw, 0};
-        }
-        e = gemm_apply<T>(c, nseg, ap, nb, nch, true);
     }
     timer_end(c, "proj_apply");
     return e;

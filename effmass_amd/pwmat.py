@@ -12,6 +12,12 @@ src/H_psi/hamiltonian.cpp:131-153) and the setup its `finalize()` does
   OUT.VR/RHO   header (n1,n2,n3,nnode[,nstate]), AL, per node a slice of
                the flat [i][j][k] grid in Hartree (src/io/RHO.cppm:57-61,
                143-193).
+  OUT.WG       header (n1,n2,n3,nband,mg_nx,nnode,nkpt,is_SO,islda), Ecut,
+               AL, ngtotnod, then per k, node and band one record of mg_nx
+               complex128 coefficients (first ngtotnod valid); with spin-
+               orbit the record holds the up half then the down half and
+               the header's mg_nx / ngtotnod count both (src/io/WG.cppm:
+               251-277, 280-307, 320-337, 360-411).
   OUT.EIGEN    header, per spin and k a (islda, ikpt, weight, k) record and
                the eigenvalues in eV (src/io/EIGEN.cppm:164-232).
   atom.config  natom, LATTICE rows (Angstrom), POSITION Z + fractional xyz
@@ -154,6 +160,154 @@ def integer_indices(K, A, kfrac) -> np.ndarray:
     halves away from zero."""
     c = (K @ A.T) / (2.0 * np.pi) - kfrac[None, :]
     return (np.sign(c) * np.floor(np.abs(c) + 0.5)).astype(np.int32)
+
+
+# --------------------------------------------------------------------------
+#  OUT.WG
+# --------------------------------------------------------------------------
+@dataclass
+class WGMeta:
+    n: tuple
+    nband: int
+    mg_nx: int              # per spin half
+    nnode: int
+    nkpt: int
+    is_so: int
+    islda: int
+    ecut: float
+    A: np.ndarray           # bohr, rows
+    ng_per_kpt: np.ndarray  # coefficients per k-point (per spin half)
+
+
+class WGFile:
+    """OUT.WG, memory-mapped: the record offsets of every (k, node, band) are
+    indexed once (WG::computeOffsets, src/io/WG.cppm:320-337); bands are
+    read on demand.  Coefficients are in the order of the matching OUT.GKK
+    k-point (nodes concatenated)."""
+
+    def __init__(self, path):
+        self._mm = np.memmap(path, dtype=np.uint8, mode="r")
+        pos = 0
+
+        def rec(p):
+            if p + 4 > self._mm.size:
+                raise RuntimeError("Failed to read record length marker")
+            (n,) = struct.unpack_from("<i", self._mm, p)
+            end = p + 4 + n
+            if n < 0 or end + 4 > self._mm.size:
+                raise RuntimeError("record extends past end of file")
+            (n2,) = struct.unpack_from("<i", self._mm, end)
+            if n2 != n:
+                raise RuntimeError("Record length mismatch")
+            return p + 4, n, end + 4
+
+        d, n, pos = rec(pos)
+        if n < 36:
+            raise RuntimeError("header: read failed")
+        n1, n2, n3, nband, mg_nx, nnode, nkpt, is_so, islda = struct.unpack_from("<9i", self._mm, d)
+        if is_so == 1:
+            mg_nx //= 2
+        d, n, pos = rec(pos)
+        (ecut,) = struct.unpack_from("<d", self._mm, d)
+        d, n, pos = rec(pos)
+        A = _lattice_from_al(bytes(self._mm[d:d + 72]))
+        d, n, pos = rec(pos)
+        ng = np.frombuffer(bytes(self._mm[d:d + n]), dtype="<i4")
+        if ng[0] != nnode:
+            raise RuntimeError("nnodes mismatch")
+        ngt = ng[1:1 + nkpt * nnode].reshape(nkpt, nnode).astype(np.int64)
+        if is_so == 1:
+            ngt //= 2
+        self.meta = WGMeta(n=(n1, n2, n3), nband=nband, mg_nx=mg_nx, nnode=nnode, nkpt=nkpt, is_so=is_so,
+                           islda=islda, ecut=ecut, A=A, ng_per_kpt=ngt.sum(axis=1))
+        self.ngtotnod = ngt
+        # file layout per k: node 0 bands 0..nband-1, node 1 bands ..., ...
+        self._off = np.zeros((nkpt, nnode, nband), dtype=np.int64)
+        self._len = np.zeros((nkpt, nnode, nband), dtype=np.int64)
+        for ik in range(nkpt):
+            for inode in range(nnode):
+                for ib in range(nband):
+                    d, n, pos = rec(pos)
+                    self._off[ik, inode, ib] = d
+                    self._len[ik, inode, ib] = n
+
+    def band(self, ikpt: int, iband: int):
+        """(up, down) coefficients of one band; down is None without
+        spin-orbit (WG::loadBand, src/io/WG.cppm:360-411)."""
+        m = self.meta
+        if ikpt < 0 or ikpt >= m.nkpt:
+            raise IndexError(f"Invalid k-point index: {ikpt}")
+        if iband < 0 or iband >= m.nband:
+            raise IndexError(f"Invalid band index: {iband}")
+        up, down = [], []
+        for inode in range(m.nnode):
+            g = int(self.ngtotnod[ikpt, inode])
+            if g == 0:
+                continue
+            o, n = int(self._off[ikpt, inode, iband]), int(self._len[ikpt, inode, iband])
+            if m.is_so == 1:
+                half = n // 32
+                if g > half:
+                    raise RuntimeError("wg: nbytes exceeds record length")
+                up.append(np.frombuffer(self._mm, dtype="<c16", count=g, offset=o))
+                down.append(np.frombuffer(self._mm, dtype="<c16", count=g, offset=o + 16 * half))
+            else:
+                if 16 * g > n:
+                    raise RuntimeError("wg: nbytes exceeds record length")
+                up.append(np.frombuffer(self._mm, dtype="<c16", count=g, offset=o))
+        u = np.concatenate(up) if up else np.zeros(0, np.complex128)
+        if m.is_so != 1:
+            return u.copy(), None
+        return u.copy(), np.concatenate(down) if down else np.zeros(0, np.complex128)
+
+    def bands(self, ikpt: int, which=None, spin: str = "up") -> np.ndarray:
+        """psi [nb, ng] (band-major, the layout effm_hpsi takes) for the
+        bands `which` (default all) of k-point ikpt."""
+        which = range(self.meta.nband) if which is None else which
+        k = 0 if spin == "up" else 1
+        rows = [self.band(ikpt, ib)[k] for ib in which]
+        if any(r is None for r in rows):
+            raise ValueError("no spin-down half without spin-orbit")
+        return np.ascontiguousarray(np.stack(rows)) if rows else np.zeros((0, 0), np.complex128)
+
+
+def write_wg(path, n, A_bohr, ecut, coeffs, node_split=None, mg_nx=None, islda=1):
+    """Write an OUT.WG in the layout WGFile reads.  coeffs[k] is [nband, ng]
+    (or an (up, down) pair of them for spin-orbit); node_split[k] lists the
+    coefficient counts per node (default one node).  Records hold mg_nx
+    (per half) complex128, zero-padded past each node's count."""
+    so = isinstance(coeffs[0], tuple)
+    nk = len(coeffs)
+    up0 = coeffs[0][0] if so else coeffs[0]
+    nband = up0.shape[0]
+    splits = node_split or [[(c[0] if so else c).shape[1]] for c in coeffs]
+    nnode = len(splits[0])
+    mg = mg_nx or max(max(s) for s in splits)
+
+    def rec(f, payload: bytes):
+        f.write(struct.pack("<i", len(payload)))
+        f.write(payload)
+        f.write(struct.pack("<i", len(payload)))
+
+    fac = 2 if so else 1
+    with open(path, "wb") as f:
+        rec(f, struct.pack("<9i", *n, nband, fac * mg, nnode, nk, 1 if so else 0, islda))
+        rec(f, struct.pack("<d", ecut))
+        rec(f, (np.asarray(A_bohr, float) * BOHR_RADIUS_ANGSTROM).astype("<f8").tobytes())
+        ngt = np.array([[fac * g for g in s] for s in splits], dtype="<i4")
+        rec(f, struct.pack("<i", nnode) + ngt.tobytes())
+        for ik in range(nk):
+            up = coeffs[ik][0] if so else coeffs[ik]
+            dn = coeffs[ik][1] if so else None
+            starts = np.concatenate([[0], np.cumsum(splits[ik])])
+            for inode in range(nnode):
+                a, b = starts[inode], starts[inode + 1]
+                for ib in range(nband):
+                    buf = np.zeros(fac * mg, dtype="<c16")
+                    buf[:b - a] = up[ib, a:b]
+                    if so:
+                        buf[mg:mg + b - a] = dn[ib, a:b]
+                    rec(f, buf.tobytes())
 
 
 # --------------------------------------------------------------------------

@@ -85,3 +85,89 @@ def test_fourier_bessel_closed_form():
     exact = np.pi ** 1.5 * q * np.exp(-q * q / 4) / 2
     assert np.abs(sp.beta_tab[0] - exact).max() < 1e-8
     assert abs(sp.dbeta_tab[0, 0] - np.pi ** 1.5 / 2) < 1e-8
+
+
+def _rec(payload: bytes) -> bytes:
+    import struct
+    return struct.pack("<i", len(payload)) + payload + struct.pack("<i", len(payload))
+
+
+def test_wg_hand_built_records(tmp_path):
+    """A two-node, two-band OUT.WG assembled record by record in the layout
+    of src/io/WG.cppm:251-277 (header), 280-307 (ngtotnod), 320-337 (per k,
+    node-major then band) and 398-406 (first ngtotnod coefficients valid)."""
+    import struct
+    # AL(3,3) column-major: AL(:, i) is lattice vector i, so the flat record
+    # is vector 0, vector 1, vector 2 (Angstrom); a hexagonal cell pins the order
+    al = np.array([[3.11, 0.0, 0.0], [-1.555, 2.6933, 0.0], [0.0, 0.0, 4.98]])
+    ng = [3, 2]                                                       # per node, mg_nx = 4
+    rng = np.random.default_rng(0)
+    c = rng.standard_normal((2, 5)) + 1j * rng.standard_normal((2, 5))
+    body = _rec(struct.pack("<9i", 16, 18, 20, 2, 4, 2, 1, 0, 1))
+    body += _rec(struct.pack("<d", 30.0))
+    body += _rec(al.astype("<f8").tobytes())
+    body += _rec(struct.pack("<3i", 2, *ng))
+    for node, (a, b) in enumerate([(0, 3), (3, 5)]):
+        for band in range(2):
+            rec = np.zeros(4, "<c16")
+            rec[:b - a] = c[band, a:b]
+            rec[b - a:] = 99.0                                         # padding is ignored
+            body += _rec(rec.tobytes())
+    p = tmp_path / "OUT.WG"
+    p.write_bytes(body)
+    w = pwmat.WGFile(str(p))
+    m = w.meta
+    assert m.n == (16, 18, 20) and m.nband == 2 and m.mg_nx == 4 and m.nnode == 2 and m.nkpt == 1
+    assert m.is_so == 0 and m.ecut == 30.0 and list(m.ng_per_kpt) == [5]
+    assert np.allclose(m.A, al / pwmat.BOHR_RADIUS_ANGSTROM)
+    up, down = w.band(0, 1)
+    assert down is None and np.array_equal(up, c[1])
+    assert np.array_equal(w.bands(0), c)
+    with pytest.raises(IndexError):
+        w.band(1, 0)
+    with pytest.raises(IndexError):
+        w.band(0, 2)
+    # a broken trailing marker is a record-length mismatch
+    bad = bytearray(body)
+    bad[-1] ^= 0x01
+    p.write_bytes(bytes(bad))
+    with pytest.raises(RuntimeError):
+        pwmat.WGFile(str(p))
+
+
+def test_wg_spin_orbit_halves_and_round_trip(tmp_path):
+    """Spin-orbit records hold the up half then the down half, each mg_nx
+    long; the header counts both halves (src/io/WG.cppm:229-249, 266-268,
+    297-299).  Also a multi-k, multi-node round trip through write_wg."""
+    rng = np.random.default_rng(1)
+    A = np.diag([10.26, 10.26, 10.26])
+    ks = []
+    splits = [[4, 3], [5, 1], [2, 6]]
+    for s in splits:
+        ng = sum(s)
+        ks.append((rng.standard_normal((3, ng)) + 1j * rng.standard_normal((3, ng)),
+                   rng.standard_normal((3, ng)) + 1j * rng.standard_normal((3, ng))))
+    p = str(tmp_path / "OUT.WG.so")
+    pwmat.write_wg(p, (24, 24, 24), A, 40.0, ks, node_split=splits, mg_nx=7)
+    w = pwmat.WGFile(p)
+    assert w.meta.is_so == 1 and w.meta.mg_nx == 7 and w.meta.nkpt == 3
+    assert list(w.meta.ng_per_kpt) == [7, 6, 8]
+    assert np.allclose(w.meta.A, A)
+    import struct
+    raw = open(p, "rb").read()
+    assert struct.unpack_from("<9i", raw, 4)[4] == 14             # file header: both halves
+    for ik in range(3):
+        for ib in range(3):
+            up, dn = w.band(ik, ib)
+            assert np.array_equal(up, ks[ik][0][ib]) and np.array_equal(dn, ks[ik][1][ib])
+        assert np.array_equal(w.bands(ik, [2, 0], spin="down"), ks[ik][1][[2, 0]])
+    # without spin-orbit
+    q = str(tmp_path / "OUT.WG")
+    plain = [k[0] for k in ks]
+    pwmat.write_wg(q, (24, 24, 24), A, 40.0, plain, node_split=splits)
+    w2 = pwmat.WGFile(q)
+    assert w2.meta.is_so == 0
+    for ik in range(3):
+        assert np.array_equal(w2.bands(ik), plain[ik])
+    with pytest.raises(ValueError):
+        w2.bands(0, spin="down")
