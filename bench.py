@@ -23,6 +23,22 @@ MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 RY_PER_HA = 2.0
 
 
+TRAFFIC_PROFILE = "profiles/hpsi_config1_traffic.md"
+
+
+def measured_traffic(kernel, prec):
+    """HBM bytes per launch of a kernel class at the 64-atom workload, from the
+    rocprofv3 PMC summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
+    plain 1 GiB device copy gives); None if the summary is absent."""
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block)[kernel][prec]["traffic_bytes"]
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def kernel_model(name, prec, nb, npw, nplanes, plane, N, nch):
     """Algorithmic bytes (or flops) of one launch of a kernel class over nb
     bands: each declared input / output counted once at storage precision."""
@@ -157,8 +173,10 @@ def main():
             kernels.append(ent)
         H.set_timing(False)
     dom = max((k for k in kernels if k["prec"] == "mixed" and "frac" in k), key=lambda k: k["share_of_step"])
+    traffic = measured_traffic(dom["kernel"], "mixed")
     roofline = {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
-                "frac": dom["frac"], "traffic": None, "kernel": dom["kernel"], "prec": "mixed"}
+                "frac": dom["frac"], "traffic": traffic, "kernel": dom["kernel"], "prec": "mixed",
+                "traffic_source": TRAFFIC_PROFILE if traffic is not None else None}
 
     # CPU baseline: the FP64 oracle (reference loop structure) on host cores
     cpu = None
