@@ -267,7 +267,7 @@ struct ProjBatch {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
 k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0) {
     using C = cx_t<T>;
     using V = typename Acc<T>::type;
@@ -285,8 +285,9 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
     C* __restrict__ part = sg.part;
     const int64_t lda = sg.lda, ldb = sg.ldb;
     const int K = sg.K, kchunk = sg.kchunk;
-    const int m0 = (t % nx) * 64 + (wave & 1) * 32;
-    const int n0 = (t / nx) * 64 + (wave >> 1) * 32;
+    const int tm0 = (t % nx) * 64, tn0 = (t / nx) * 64;      // workgroup tile
+    const int m0 = tm0 + (wave & 1) * 32;                     // wave tile
+    const int n0 = tn0 + (wave >> 1) * 32;
     const int ks = split * kchunk;
     const int ke = min(K, ks + kchunk);
     // Gauss's three-product complex MAC: for conj(P) psi with a = Re P,
@@ -300,66 +301,98 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
             k2[r][q] = V{0, 0, 0, 0};
             k3[r][q] = V{0, 0, 0, 0};
         }
+    // Slabs of BK k (128 bytes of a row) of the tile's 64 rows of A and 64 of
+    // B are staged through LDS, double-buffered: each global load instruction
+    // covers whole cache lines (consecutive lanes, consecutive k), and every
+    // element is fetched once per workgroup instead of once per wave.
+    constexpr int BK = 128 / (int)sizeof(C);          // FP32 16, FP64 8
+    constexpr int LDK = BK + 32 / (int)sizeof(C);     // row stride, 32-byte pad: conflict-free 16-byte reads
+    constexpr int EPT = 64 * BK / 256;                // elements per thread, matrix and slab
+    constexpr int KPL = BK / 4;                       // consecutive k per lane and slab
+    __shared__ __align__(16) C sA[2][64 * LDK];
+    __shared__ __align__(16) C sB[2][64 * LDK];
     // Rows / columns past M / N are clamped onto the last one: they only feed
     // outputs that are never stored, so the loads need no branch.
-    const C* ar[2];
-    const C* br[2];
-    for (int r = 0; r < 2; ++r) {
-        ar[r] = A + (int64_t)min(m0 + r * 16 + i, M - 1) * lda;
-        br[r] = B + (int64_t)min(n0 + r * 16 + i, N - 1) * ldb;
+    const C* ga[EPT];
+    const C* gb[EPT];
+    int lofs[EPT], kofs[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = threadIdx.x + 256 * q;
+        const int row = e / BK, col = e - row * BK;
+        ga[q] = A + (int64_t)min(tm0 + row, M - 1) * lda;
+        gb[q] = B + (int64_t)min(tn0 + row, N - 1) * ldb;
+        lofs[q] = row * LDK + col;
+        kofs[q] = col;
     }
-    // register double buffering: the next 16-wide k slab is loaded while
-    // the MFMAs of the current one run
-    C a[2][2][4], b[2][2][4];
+    C pa[EPT], pb[EPT];
     // k past the split end reads a clamped address and is zeroed by a select
-    auto load = [&](int buf, int kb) {
-        const int kl = kb + kk * 4;
+    auto gload = [&](int kb) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const bool kv = (kl + s) < ke;
-            const int kc = kv ? kl + s : ke - 1;
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                const C x = ar[r][kc];
-                const C y = br[r][kc];
-                a[buf][r][s] = kv ? x : C{0, 0};
-                b[buf][r][s] = y;
-            }
+        for (int q = 0; q < EPT; ++q) {
+            const int k = kb + kofs[q];
+            const bool kv = k < ke;
+            const int kc = kv ? k : ke - 1;
+            const C x = ga[q][kc];
+            const C y = gb[q][kc];
+            pa[q] = kv ? x : C{0, 0};
+            pb[q] = y;
         }
     };
-    auto mma = [&](int buf) {
+    auto lstore = [&](int buf) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
+        for (int q = 0; q < EPT; ++q) {
+            sA[buf][lofs[q]] = pa[q];
+            sB[buf][lofs[q]] = pb[q];
+        }
+    };
+    // lane (i, kk) takes k = kk*KPL + s of rows / columns i of its blocks
+    auto mma = [&](int buf) {
+        C a[2][KPL], b[2][KPL];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int s = 0; s < KPL; ++s) {
+                a[r][s] = sA[buf][(m0 - tm0 + r * 16 + i) * LDK + kk * KPL + s];
+                b[r][s] = sB[buf][(n0 - tn0 + r * 16 + i) * LDK + kk * KPL + s];
+            }
+#pragma unroll
+        for (int s = 0; s < KPL; ++s) {
             T sa[2], sc[2], dc[2];
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
-                sa[r] = a[buf][r][s].x - a[buf][r][s].y;             // a + b
-                sc[r] = b[buf][r][s].x + b[buf][r][s].y;             // c + d
-                dc[r] = b[buf][r][s].y - b[buf][r][s].x;             // d - c
+                sa[r] = a[r][s].x - a[r][s].y;             // a + b
+                sc[r] = b[r][s].x + b[r][s].y;             // c + d
+                dc[r] = b[r][s].y - b[r][s].x;             // d - c
             }
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) k1[r][q] = mfma16(sa[r], b[buf][q][s].x, k1[r][q]);
+                for (int q = 0; q < 2; ++q) k1[r][q] = mfma16(sa[r], b[q][s].x, k1[r][q]);
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) k2[r][q] = mfma16(a[buf][r][s].x, dc[q], k2[r][q]);
+                for (int q = 0; q < 2; ++q) k2[r][q] = mfma16(a[r][s].x, dc[q], k2[r][q]);
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) k3[r][q] = mfma16(-a[buf][r][s].y, sc[q], k3[r][q]);
+                for (int q = 0; q < 2; ++q) k3[r][q] = mfma16(-a[r][s].y, sc[q], k3[r][q]);
         }
     };
-    if (ks < ke) load(0, ks);
-    int kb = ks;
-    for (; kb + 16 < ke; kb += 32) {
-        load(1, kb + 16);
-        mma(0);
-        if (kb + 32 < ke) load(0, kb + 32);
-        mma(1);
+    if (ks < ke) {
+        gload(ks);
+        lstore(0);
+        __syncthreads();
+        int buf = 0;
+        for (int kb = ks; kb < ke; kb += BK) {
+            const bool more = kb + BK < ke;
+            if (more) gload(kb + BK);          // next slab in flight during the MFMAs
+            mma(buf);
+            if (more) lstore(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
     }
-    if (kb < ke) mma(0);
     C* po = part + (int64_t)split * Mtot * N;
     for (int r = 0; r < 2; ++r)
         for (int q = 0; q < 2; ++q)
