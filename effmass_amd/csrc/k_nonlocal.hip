@@ -306,7 +306,9 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
     // covers whole cache lines (consecutive lanes, consecutive k), and every
     // element is fetched once per workgroup instead of once per wave.
     constexpr int BK = 128 / (int)sizeof(C);          // FP32 16, FP64 8
-    constexpr int LDK = BK + 32 / (int)sizeof(C);     // row stride, 32-byte pad: conflict-free 16-byte reads
+    // row stride of 144 bytes (36 banks): the 16 lanes of one 16-byte LDS
+    // read cycle hit 16 distinct 4-bank groups
+    constexpr int LDK = BK + 16 / (int)sizeof(C);
     constexpr int EPT = 64 * BK / 256;                // elements per thread, matrix and slab
     constexpr int KPL = BK / 4;                       // consecutive k per lane and slab
     __shared__ __align__(16) C sA[2][64 * LDK];
@@ -461,7 +463,7 @@ struct ApplyBatch {
 };
 
 template <typename T, bool TWO>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
 k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     using C = cx_t<T>;
     using V = typename Acc<T>::type;
@@ -484,8 +486,9 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     const C* __restrict__ src = sg.src;
     const int64_t lda = sg.lda, ldb = sg.ldb, ldo = sg.ldo, lds = sg.lds;
     const int M = sg.M;
-    const int m0 = (L / ny) * 64 + (wave & 1) * 32;
-    const int n0 = (L % ny) * 64 + (wave >> 1) * 32;
+    const int tm0 = (L / ny) * 64, tn0 = (L % ny) * 64;      // workgroup tile
+    const int m0 = tm0 + (wave & 1) * 32;                     // wave tile
+    const int n0 = tn0 + (wave >> 1) * 32;
     // Gauss's three-product complex MAC with a = Re P, b = Im P, c = Re C,
     // d = Im C: k1 += c (a + b), k2 += a (d - c), k3 += b (c + d).
     V k1[2][2], k2[2][2], k3[2][2];
@@ -495,66 +498,93 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
             k2[r][q] = V{0, 0, 0, 0};
             k3[r][q] = V{0, 0, 0, 0};
         }
-    // clamped columns of A / B (see k_gemm_proj)
-    int mc[2], nc[2];
-    for (int r = 0; r < 2; ++r) {
-        mc[r] = min(m0 + r * 16 + i, M - 1);
-        nc[r] = min(n0 + r * 16 + i, N - 1);
-    }
-    // four k-steps of 4 per slab, double buffered in registers
-    C a[2][4][2], b[2][4][2];
-    auto load = [&](int buf, int kb) {
+    // Slabs of BK k-rows of A (64 consecutive m) and of B (64 consecutive n)
+    // are staged through LDS, double-buffered; each global load instruction
+    // reads one 64-element row segment (whole cache lines).
+    constexpr int BK = 128 / (int)sizeof(C);          // FP32 16, FP64 8
+    // row stride of 64 elements + 128 bytes: consecutive k-rows start 32
+    // banks apart, so the two 16-lane halves of an 8-byte read cycle do not
+    // collide
+    constexpr int LDM = 64 + 128 / (int)sizeof(C);
+    constexpr int EPT = 64 * BK / 256;                // elements per thread, matrix and slab
+    __shared__ __align__(16) C sA[2][BK * LDM];
+    __shared__ __align__(16) C sB[2][BK * LDM];
+    // columns past M / N are clamped onto the last one: they only feed
+    // outputs that are never stored
+    const int lrow = threadIdx.x >> 6, lcol = threadIdx.x & 63;
+    const int mcl = min(tm0 + lcol, M - 1), ncl = min(tn0 + lcol, N - 1);
+    C pa[EPT], pb[EPT];
+    // k past the end reads a clamped address and is zeroed by a select;
+    // TWO: k in [K, 2K) reads row k - K of A2 / B2
+    auto gload = [&](int kb) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int k = kb + 4 * s + kk;
+        for (int q = 0; q < EPT; ++q) {
+            const int k = kb + lrow + 4 * q;
             const bool kv = k < Ktot;
             const int kt = kv ? k : Ktot - 1;
             const bool sec = TWO && kt >= K;
             const int kc = sec ? kt - K : kt;
             const C* Ab = sec ? A2 : A;
             const C* Bb = sec ? B2 : B;
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                const C x = Ab[(int64_t)kc * lda + mc[r]];
-                const C y = Bb[(int64_t)kc * ldb + nc[r]];
-                a[buf][s][r] = kv ? x : C{0, 0};
-                b[buf][s][r] = y;
-            }
+            const C x = Ab[(int64_t)kc * lda + mcl];
+            const C y = Bb[(int64_t)kc * ldb + ncl];
+            pa[q] = kv ? x : C{0, 0};
+            pb[q] = y;
         }
     };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            sA[buf][(lrow + 4 * q) * LDM + lcol] = pa[q];
+            sB[buf][(lrow + 4 * q) * LDM + lcol] = pb[q];
+        }
+    };
+    // MFMA k-step s of the slab: lane (i, kk) takes k-row 4 s + kk
     auto mma = [&](int buf) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
+        for (int s = 0; s < BK / 4; ++s) {
+            const int kr = 4 * s + kk;
+            C a[2], b[2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                a[r] = sA[buf][kr * LDM + (m0 - tm0) + r * 16 + i];
+                b[r] = sB[buf][kr * LDM + (n0 - tn0) + r * 16 + i];
+            }
             T sa[2], sc[2], dc[2];
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
-                sa[r] = a[buf][s][r].x + a[buf][s][r].y;
-                sc[r] = b[buf][s][r].x + b[buf][s][r].y;
-                dc[r] = b[buf][s][r].y - b[buf][s][r].x;
+                sa[r] = a[r].x + a[r].y;
+                sc[r] = b[r].x + b[r].y;
+                dc[r] = b[r].y - b[r].x;
             }
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) k1[r][q] = mfma16(sa[r], b[buf][s][q].x, k1[r][q]);
+                for (int q = 0; q < 2; ++q) k1[r][q] = mfma16(sa[r], b[q].x, k1[r][q]);
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) k2[r][q] = mfma16(a[buf][s][r].x, dc[q], k2[r][q]);
+                for (int q = 0; q < 2; ++q) k2[r][q] = mfma16(a[r].x, dc[q], k2[r][q]);
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) k3[r][q] = mfma16(a[buf][s][r].y, sc[q], k3[r][q]);
+                for (int q = 0; q < 2; ++q) k3[r][q] = mfma16(a[r].y, sc[q], k3[r][q]);
         }
     };
-    if (Ktot > 0) load(0, 0);
-    int kb = 0;
-    for (; kb + 16 < Ktot; kb += 32) {
-        load(1, kb + 16);
-        mma(0);
-        if (kb + 32 < Ktot) load(0, kb + 32);
-        mma(1);
+    if (Ktot > 0) {
+        gload(0);
+        lstore(0);
+        __syncthreads();
+        int buf = 0;
+        for (int kb = 0; kb < Ktot; kb += BK) {
+            const bool more = kb + BK < Ktot;
+            if (more) gload(kb + BK);          // next slab in flight during the MFMAs
+            mma(buf);
+            if (more) lstore(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
     }
-    if (kb < Ktot) mma(0);
     for (int r = 0; r < 2; ++r)
         for (int q = 0; q < 2; ++q) {
             const int n = n0 + q * 16 + i;
