@@ -184,7 +184,7 @@ static void timer_resolve(Ctx& c) {
 
 static void release_kpt(Ctx& c, KptDev& k) {
     for (DevBuf* b : {&k.K, &k.kin, &k.gidx, &k.plane_x, &k.x_to_plane, &k.plane_ptr, &k.ent_ig,
-                      &k.ent_pos, &k.P64, &k.P32, &k.dP64, &k.dP32})
+                      &k.ent_pos, &k.row_ptr, &k.rows, &k.P64, &k.P32, &k.dP64, &k.dP32})
         dfree(c, *b);
     k = KptDev{};
 }
@@ -709,6 +709,18 @@ int effm_kpt_upload(effm_ctx* h, int ikpt, int npw, const double* K, const doubl
         ent_ig[e] = g;
         ent_pos[e] = pos[g];
     }
+    // per plane: the (digit-reversed) rows that hold sphere points
+    std::vector<int> row_ptr(np + 1, 0), rows;
+    {
+        std::vector<char> has((size_t)n2, 0);
+        for (int p = 0; p < np; ++p) {
+            std::fill(has.begin(), has.end(), 0);
+            for (int e = ptr[p]; e < ptr[p + 1]; ++e) has[ent_pos[e] / n3] = 1;
+            for (int r = 0; r < n2; ++r)
+                if (has[r]) rows.push_back(r);
+            row_ptr[p + 1] = (int)rows.size();
+        }
+    }
     if (ikpt >= (int)c.kpts.size()) c.kpts.resize(ikpt + 1);
     KptDev& k = c.kpts[ikpt];
     release_kpt(c, k);
@@ -728,6 +740,8 @@ int effm_kpt_upload(effm_ctx* h, int ikpt, int npw, const double* K, const doubl
     if ((rc = upload(c, k.plane_ptr, ptr, "plane ptr"))) return rc;
     if ((rc = upload(c, k.ent_ig, ent_ig, "plane entries"))) return rc;
     if ((rc = upload(c, k.ent_pos, ent_pos, "plane positions"))) return rc;
+    if ((rc = upload(c, k.row_ptr, row_ptr, "plane row ptr"))) return rc;
+    if ((rc = upload(c, k.rows, rows, "plane rows"))) return rc;
     k.nplanes = np;
     int mx = 0;
     for (int p = 0; p < np; ++p) mx = std::max(mx, ptr[p + 1] - ptr[p]);

@@ -69,6 +69,8 @@ struct KptDev {
     DevBuf plane_ptr;    // int[nplanes+1]
     DevBuf ent_ig;       // int[npw]
     DevBuf ent_pos;      // int[npw] digit-reversed (j,k) position in the plane
+    DevBuf row_ptr;      // int[nplanes+1]: per plane, its rows holding sphere points
+    DevBuf rows;         //   (digit-reversed row index j', ascending)
     int max_plane_entries = 0;
     // projector matrices, built on first use: P[c][g]; dP[alpha][c][g]
     DevBuf P64, P32, dP64, dP32;

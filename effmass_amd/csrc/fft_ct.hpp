@@ -65,6 +65,46 @@ __device__ __forceinline__ void ct_fft(cx_t<T>* buf, const cx_t<T>* tw) {
     }
 }
 
+// The contiguous-axis transform of only the listed rows of an N x N plane
+// (rows[0 .. nrows) are row indices, e.g. in LDS): same stages as ct_fft
+// with ES = 1, LS = N.  Rows that hold no data (G2R) or whose results are
+// not gathered (R2G) are skipped.  Ends with a barrier.
+template <int N, int P, int L, int S, bool DIT, int BLOCK, typename T>
+__device__ __forceinline__ void ct_stage_rows(cx_t<T>* buf, const cx_t<T>* tw, const int* rows, int nrows) {
+    constexpr int NBF = N / P;
+    constexpr int LT = L * P;
+    constexpr int TSTEP = N / LT;
+    constexpr int ITER = (N * NBF + BLOCK - 1) / BLOCK;
+    const int total = nrows * NBF;
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+        const int t = threadIdx.x + it * BLOCK;
+        if (t < total) {
+            const int li = t / NBF;
+            const int r = t - li * NBF;
+            const int b = r / L;
+            const int k1 = r - b * L;
+            butterfly<P, S, DIT, T>(buf, rows[li] * N + b * LT + k1, L, k1, TSTEP, tw);
+        }
+    }
+}
+
+template <int N, int S, bool DIT, int BLOCK, typename T>
+__device__ __forceinline__ void ct_fft_rows(cx_t<T>* buf, const cx_t<T>* tw, const int* rows, int nrows) {
+    constexpr int P1 = Plan2<N>::P1, P2 = Plan2<N>::P2;
+    if constexpr (DIT) {
+        ct_stage_rows<N, P1, 1, S, true, BLOCK, T>(buf, tw, rows, nrows);
+        __syncthreads();
+        ct_stage_rows<N, P2, P1, S, true, BLOCK, T>(buf, tw, rows, nrows);
+        __syncthreads();
+    } else {
+        ct_stage_rows<N, P2, P1, S, false, BLOCK, T>(buf, tw, rows, nrows);
+        __syncthreads();
+        ct_stage_rows<N, P1, 1, S, false, BLOCK, T>(buf, tw, rows, nrows);
+        __syncthreads();
+    }
+}
+
 constexpr int round_up(int a, int b) { return (a + b - 1) / b * b; }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int cmin(int a, int b) { return a < b ? a : b; }

@@ -41,6 +41,8 @@ struct LocalSeg {
     const int* ent_pos;       // [npw]
     const int* x_to_plane;    // [n1]
     const int* plane_x;       // [nplanes]
+    const int* row_ptr;       // [nplanes + 1]
+    const int* rows;          // rows of each plane that hold sphere points
     const double* kin;        // [npw]
     cx_t<T>* work;            // [nb][nplanes][ws_plane]
     int nplanes;
@@ -195,12 +197,13 @@ k_plane_r2g(const LocalBatch<T> bt, T scale, int accumulate, FftAxis a2, FftAxis
 //  with compile-time two-stage plans (fft_ct.hpp).  Same data flow as above.
 // ---------------------------------------------------------------------------
 // Persistent plane kernels: each workgroup walks the flat (segment, plane,
-// band) items with stride gridDim.x.  In FP32 the next item's global loads
-// are issued before this item's global stores, so both drain behind the
-// next item's LDS work; in FP64 the registers that would hold them are
-// taken by the FFT itself.
+// band) items with stride gridDim.x.  The R2G pass in FP32 loads the next
+// item's plane into registers during this item's FFT; in FP64 those
+// registers are taken by the FFT itself.  (Measured on the 90^3 grid: the
+// same overlap in the G2R pass, the next item's scatter loads issued before
+// this item's stores, made that pass 30% slower, so G2R does not pipeline.)
 template <typename T>
-constexpr bool kPipePlanes = sizeof(T) == 4;
+constexpr bool kPipeR2G = sizeof(T) == 4;
 
 template <typename T, int N, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
@@ -211,85 +214,40 @@ k_plane_g2r_ct(const LocalBatch<T> bt, int nitems, const cx_t<T>* __restrict__ t
     constexpr int NP = N * WP;                          // rows written (pad columns included)
     constexpr int64_t PS = ws_plane<T>(N, N);           // workspace plane stride
     constexpr int NOUT = (NP + BLOCK - 1) / BLOCK;
-    constexpr int PF = 2;                               // prefetched scatter entries per thread
+    constexpr int NZ = (N2 + BLOCK - 1) / BLOCK;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* plane = reinterpret_cast<C*>(smem_raw);
     C* tw = plane + N2;
+    int* srows = reinterpret_cast<int*>(tw + N);        // rows of the plane holding data
     if (threadIdx.x < N) tw[threadIdx.x] = tw_g[threadIdx.x];
-    // FP64 prefetches nothing (no registers to spare next to the FFT)
-    constexpr int NPF = kPipePlanes<T> ? PF : 0;
-    C pv[PF];
-    int pp[PF];
-    int e0 = 0, e1 = 0;
-    auto prefetch = [&](int it) {
-        const LocalSeg<T>& sg = bt.s[seg_of(bt, it)];
-        const int local = it - sg.item0;
-        const int p = local % sg.nplanes, b = local / sg.nplanes;
-        e0 = sg.plane_ptr[p];
-        e1 = sg.plane_ptr[p + 1];
-#pragma unroll
-        for (int f = 0; f < NPF; ++f) {
-            const int e = e0 + threadIdx.x + f * BLOCK;
-            pp[f] = -1;
-            if (e < e1) {
-                pp[f] = sg.ent_pos[e];
-                pv[f] = sg.psi[(int64_t)b * sg.ld + sg.ent_ig[e]];
-            }
-        }
-    };
-    int item = blockIdx.x;
-    if (item < nitems) prefetch(item);
-    for (; item < nitems; item += gridDim.x) {
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
         const LocalSeg<T>& sg = bt.s[seg_of(bt, item)];
         const int local = item - sg.item0;
         const int p = local % sg.nplanes, b = local / sg.nplanes;
-        constexpr int NZ = (N2 + BLOCK - 1) / BLOCK;
+        const int r0 = sg.row_ptr[p], nrows = sg.row_ptr[p + 1] - r0;
+        if (threadIdx.x < nrows) srows[threadIdx.x] = sg.rows[r0 + threadIdx.x];
 #pragma unroll
         for (int it = 0; it < NZ; ++it) {
             const int i = threadIdx.x + it * BLOCK;
             if (NZ * BLOCK == N2 || i < N2) plane[i] = C{0, 0};
         }
         __syncthreads();
-#pragma unroll
-        for (int f = 0; f < NPF; ++f)
-            if (pp[f] >= 0) plane[pp[f]] = pv[f];
-        for (int e = e0 + threadIdx.x + NPF * BLOCK; e < e1; e += BLOCK)
-            plane[sg.ent_pos[e]] = sg.psi[(int64_t)b * sg.ld + sg.ent_ig[e]];
+        const C* ps = sg.psi + (int64_t)b * sg.ld;
+        const int e0 = sg.plane_ptr[p], e1 = sg.plane_ptr[p + 1];
+        for (int e = e0 + threadIdx.x; e < e1; e += BLOCK) plane[sg.ent_pos[e]] = ps[sg.ent_ig[e]];
         __syncthreads();
-        ct_fft<N, +1, true, 1, N, N, BLOCK, T>(plane, tw);     // along k (rows)
-        ct_fft<N, +1, true, N, 1, N, BLOCK, T>(plane, tw);     // along j (columns)
+        ct_fft_rows<N, +1, true, BLOCK, T>(plane, tw, srows, nrows);   // along k, rows with data
+        ct_fft<N, +1, true, N, 1, N, BLOCK, T>(plane, tw);             // along j (columns)
         C* o = sg.work + ((int64_t)b * sg.nplanes + p) * PS;
-        const int nxt = item + gridDim.x;
-        if constexpr (kPipePlanes<T>) {
-            // FP32: plane -> registers, then the next item's loads, then the stores
-            C ov[NOUT];
 #pragma unroll
-            for (int it = 0; it < NOUT; ++it) {
-                const int idx = threadIdx.x + it * BLOCK;
-                const int j = idx / WP;
-                const int kk = idx - j * WP;
-                ov[it] = plane[min(j, N - 1) * N + min(kk, N - 1)];   // pad columns carry junk
-            }
-            __syncthreads();
-            if (nxt < nitems) prefetch(nxt);
-#pragma unroll
-            for (int it = 0; it < NOUT; ++it) {
-                const int idx = threadIdx.x + it * BLOCK;
-                if (NOUT * BLOCK == NP || idx < NP) o[idx] = ov[it];
-            }
-        } else {
-            // FP64: a plane of registers does not fit next to the FFT's; store
-            // straight from LDS, then issue the next item's loads
-#pragma unroll
-            for (int it = 0; it < NOUT; ++it) {
-                const int idx = threadIdx.x + it * BLOCK;
-                const int j = idx / WP;
-                const int kk = idx - j * WP;
-                if (NOUT * BLOCK == NP || idx < NP) o[idx] = plane[min(j, N - 1) * N + min(kk, N - 1)];
-            }
-            __syncthreads();
-            if (nxt < nitems) prefetch(nxt);
+        for (int it = 0; it < NOUT; ++it) {
+            const int idx = threadIdx.x + it * BLOCK;
+            const int j = idx / WP;
+            const int kk = idx - j * WP;
+            // pad columns of the row (FP32 rows of 128-byte pitch) carry junk
+            if (NOUT * BLOCK == NP || idx < NP) o[idx] = plane[min(j, N - 1) * N + min(kk, N - 1)];
         }
+        __syncthreads();
     }
 }
 
@@ -307,6 +265,7 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* plane = reinterpret_cast<C*>(smem_raw);
     C* tw = plane + N2;
+    int* srows = reinterpret_cast<int*>(tw + N);        // rows of the plane that are gathered
     if (threadIdx.x < N) tw[threadIdx.x] = tw_g[threadIdx.x];
     auto plane_in = [&](int it) {
         const LocalSeg<T>& sg = bt.s[seg_of(bt, it)];
@@ -316,7 +275,7 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
     };
     // FP32: the next plane is prefetched into registers behind this one's
     // FFT; FP64 loads each plane at the top of its item
-    constexpr int NIV = kPipePlanes<T> ? NIN : 1;
+    constexpr int NIV = kPipeR2G<T> ? NIN : 1;
     C iv[NIV];
     auto prefetch = [&](int it) {
         const C* in = plane_in(it);
@@ -333,13 +292,13 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
         if (kk < N && j < N) plane[j * N + kk] = v;
     };
     int item = blockIdx.x;
-    if constexpr (kPipePlanes<T>)
+    if constexpr (kPipeR2G<T>)
         if (item < nitems) prefetch(item);
     for (; item < nitems; item += gridDim.x) {
         const LocalSeg<T>& sg = bt.s[seg_of(bt, item)];
         const int local = item - sg.item0;
         const int p = local % sg.nplanes, b = local / sg.nplanes;
-        if constexpr (kPipePlanes<T>) {
+        if constexpr (kPipeR2G<T>) {
 #pragma unroll
             for (int q = 0; q < NIN; ++q) to_lds(q, iv[q]);
         } else {
@@ -347,6 +306,8 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
 #pragma unroll
             for (int q = 0; q < NIN; ++q) to_lds(q, in[min((int)threadIdx.x + q * BLOCK, NP - 1)]);
         }
+        const int r0 = sg.row_ptr[p], nrows = sg.row_ptr[p + 1] - r0;
+        if (threadIdx.x < nrows) srows[threadIdx.x] = sg.rows[r0 + threadIdx.x];
         __syncthreads();
         const int e0 = sg.plane_ptr[p], e1 = sg.plane_ptr[p + 1];
         const C* ps = sg.psi + (int64_t)b * sg.ld;
@@ -367,7 +328,7 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
             }
             hs[ig] = C{(T)re, (T)im};
         };
-        if constexpr (kPipePlanes<T>) {
+        if constexpr (kPipeR2G<T>) {
             // this item's gather inputs, then the next plane, both in flight
             // during the FFT
             int gi[PF], gp[PF];
@@ -387,8 +348,8 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
             }
             const int nxt = item + gridDim.x;
             if (nxt < nitems) prefetch(nxt);
-            ct_fft<N, -1, false, N, 1, N, BLOCK, T>(plane, tw);    // along j
-            ct_fft<N, -1, false, 1, N, N, BLOCK, T>(plane, tw);    // along k
+            ct_fft<N, -1, false, N, 1, N, BLOCK, T>(plane, tw);          // along j
+            ct_fft_rows<N, -1, false, BLOCK, T>(plane, tw, srows, nrows); // along k, gathered rows
 #pragma unroll
             for (int f = 0; f < PF; ++f) {
                 if (gi[f] < 0) continue;
@@ -399,8 +360,8 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
             }
             for (int e = e0 + threadIdx.x + PF * BLOCK; e < e1; e += BLOCK) gather(e);
         } else {
-            ct_fft<N, -1, false, N, 1, N, BLOCK, T>(plane, tw);    // along j
-            ct_fft<N, -1, false, 1, N, N, BLOCK, T>(plane, tw);    // along k
+            ct_fft<N, -1, false, N, 1, N, BLOCK, T>(plane, tw);          // along j
+            ct_fft_rows<N, -1, false, BLOCK, T>(plane, tw, srows, nrows); // along k, gathered rows
             for (int e = e0 + threadIdx.x; e < e1; e += BLOCK) gather(e);
         }
         __syncthreads();
@@ -558,7 +519,7 @@ template <typename T, int N>
 static hipError_t launch_planes_ct(Ctx& c, const LocalBatch<T>& bt, int nitems, bool accumulate,
                                    bool g2r, const cx_t<T>* tw) {
     constexpr int BLOCK = plane_block<N>();
-    const size_t lds = sizeof(cx_t<T>) * ((size_t)N * N + N);
+    const size_t lds = sizeof(cx_t<T>) * ((size_t)N * N + N) + sizeof(int) * N;
     hipError_t e;
     int grid = 0;
     if (g2r) {
@@ -634,6 +595,8 @@ hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, 
         s.ent_pos = static_cast<const int*>(k.ent_pos.ptr);
         s.x_to_plane = static_cast<const int*>(k.x_to_plane.ptr);
         s.plane_x = static_cast<const int*>(k.plane_x.ptr);
+        s.row_ptr = static_cast<const int*>(k.row_ptr.ptr);
+        s.rows = static_cast<const int*>(k.rows.ptr);
         s.kin = static_cast<const double*>(k.kin.ptr);
         s.work = wcur;
         s.nplanes = k.nplanes;
