@@ -217,8 +217,7 @@ def main():
         from effmass_amd import serving
         torch.cuda.empty_cache()
         qs = serving.make_stream(args.serve_queries, rate=50.0, seed=2024)
-        serving.run_query(serving.make_stream(1, seed=7)[0])       # warm-up (kernel loading)
-        _, serving_stats = serving.serve(qs, slots=args.serve_slots, prec="mixed")
+        _, serving_stats = serving.serve(qs, slots=args.serve_slots, prec="mixed", timeout=900)
         serving_stats["rate_per_s"] = 50.0
         serving_stats["workload"] = ("Poisson 50/s; cells of 2/8/16/32/64 atoms at 135 bohr^3/atom, "
                                      "ecut 30/40/50 Ry, 16-256 bands, 1-4 k-points; m* of band nb/2 "

@@ -41,9 +41,13 @@ static int hip_fail(Ctx& c, hipError_t e, const char* where) {
         if (_e != hipSuccess) return hip_fail(c, _e, where); \
     } while (0)
 
+// Device buffers are stream-ordered (hipMallocAsync / hipFreeAsync on the
+// context's stream): a free waits only for this stream's earlier work, not
+// for the whole device, so contexts that come and go (one per served query)
+// do not stall the others.
 static void dfree(Ctx& c, DevBuf& b) {
     if (b.ptr) {
-        (void)hipFree(b.ptr);
+        (void)hipFreeAsync(b.ptr, c.stream);
         c.used -= b.bytes;
         ++c.epoch;
     }
@@ -58,10 +62,12 @@ static int dalloc(Ctx& c, DevBuf& b, size_t bytes, const char* what) {
         return fail(c, kErrNoMem, std::string("HBM budget exceeded allocating ") + what + " (" +
                                       std::to_string(bytes) + " B, in use " + std::to_string(c.used) +
                                       " B, budget " + std::to_string(c.budget) + " B)");
-    hipError_t e = hipMalloc(&b.ptr, bytes);
+    hipError_t e = hipMallocAsync(&b.ptr, bytes, c.stream);
+    // the buffer is used right away, also by synchronous copies
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
     if (e != hipSuccess) {
         b.ptr = nullptr;
-        return fail(c, kErrNoMem, std::string("hipMalloc failed for ") + what + ": " + hipGetErrorString(e));
+        return fail(c, kErrNoMem, std::string("hipMallocAsync failed for ") + what + ": " + hipGetErrorString(e));
     }
     b.bytes = bytes;
     c.used += bytes;
@@ -497,13 +503,18 @@ int effm_ctx_destroy(effm_ctx* h) {
     for (auto e : c.event_pool) (void)hipEventDestroy(e);
     drop_graphs(c);
     if (c.capture_stream) (void)hipStreamDestroy(c.capture_stream);
+    (void)hipStreamSynchronize(c.stream);      // the stream-ordered frees above
     delete h;
     return kOk;
 }
 
 int effm_set_stream(effm_ctx* h, void* stream) {
     if (!h) return kErrInvalid;
-    h->c.stream = static_cast<hipStream_t>(stream);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // buffers are freed in the order of the current stream: finish the old
+    // stream's work before switching
+    if (s != h->c.stream) (void)hipStreamSynchronize(h->c.stream);
+    h->c.stream = s;
     return kOk;
 }
 
@@ -921,6 +932,29 @@ int effm_autotune(effm_ctx* h, int ikpt, int nb, int prec, int* pencil_bands, in
     KptDev& k = c.kpts[ikpt];
     return prec == EFFM_PREC_FP64 ? autotune<double>(c, k, nb, pencil_bands, gemm_wgs)
                                   : autotune<float>(c, k, nb, pencil_bands, gemm_wgs);
+}
+
+int effm_overlap(effm_ctx* h, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
+                 void* C, int64_t ldc, int prec, int flags) {
+    if (!h) return kErrInvalid;
+    Ctx& c = h->c;
+    (void)hipSetDevice(c.device);
+    if (m < 0 || n < 0 || k < 0) return fail(c, kErrInvalid, "negative overlap dimension");
+    if (m == 0 || n == 0) return kOk;
+    if (!C || (k > 0 && (!A || !B))) return fail(c, kErrInvalid, "overlap operands must not be NULL");
+    if (k > 0 && (lda < k || ldb < k)) return fail(c, kErrInvalid, "leading dimension smaller than k");
+    if (ldc < n) return fail(c, kErrInvalid, "ldc smaller than n");
+    if (prec != EFFM_PREC_FP64 && prec != EFFM_PREC_MIXED) return fail(c, kErrInvalid, "unknown precision");
+    if (flags & (EFFM_F_HOST | EFFM_F_GRAPH)) return fail(c, kErrInvalid, "effm_overlap takes device pointers only");
+    const size_t pb = prec == EFFM_PREC_FP64 ? overlap_partial_bytes<double>(c, m, n, std::max(k, 1))
+                                             : overlap_partial_bytes<float>(c, m, n, std::max(k, 1));
+    int rc = ensure(c, c.partial, pb, "overlap partial sums");
+    if (rc) return rc;
+    hipError_t e = prec == EFFM_PREC_FP64
+                       ? launch_overlap<double>(c, m, n, k, A, lda, B, ldb, C, ldc, c.partial.ptr, c.partial.bytes)
+                       : launch_overlap<float>(c, m, n, k, A, lda, B, ldb, C, ldc, c.partial.ptr, c.partial.bytes);
+    if (e != hipSuccess) return hip_fail(c, e, "overlap");
+    return kOk;
 }
 
 int effm_rayleigh(effm_ctx* h, int npw, int nb, const void* psi, int64_t ld, const void* hpsi,

@@ -237,6 +237,13 @@ hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accu
 template <typename T>
 size_t vnl_partial_bytes(Ctx& c, int nrows, int nb, int min_npw, int nseg);
 
+// overlap C = conj(A) B^T with FP64-reduced split-K partial sums
+template <typename T>
+size_t overlap_partial_bytes(Ctx& c, int m, int n, int k);
+template <typename T>
+hipError_t launch_overlap(Ctx& c, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
+                          void* C, int64_t ldc, void* partial, size_t partial_bytes);
+
 // candidate launch shapes effm_autotune times
 constexpr int kPencilBandChoices[] = {4, 8, 16};
 constexpr int kGemmWgChoices[] = {512, 768, 1024, 1536, 2048};

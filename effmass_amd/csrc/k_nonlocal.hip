@@ -418,7 +418,7 @@ struct ReduceBatch {
 // grid: nseg x per_seg blocks of 256
 template <typename T>
 __global__ void k_reduce_scale(const ReduceBatch<T> bt, int per_seg, int nsplit, int Mtot, int N,
-                               const double* __restrict__ lam, int nch) {
+                               const double* __restrict__ lam, int nch, int64_t ldo) {
     const int seg = blockIdx.x / per_seg;
     const int64_t idx = (int64_t)(blockIdx.x - seg * per_seg) * blockDim.x + threadIdx.x;
     const int64_t tot = (int64_t)Mtot * N;
@@ -432,8 +432,8 @@ __global__ void k_reduce_scale(const ReduceBatch<T> bt, int per_seg, int nsplit,
         im += (double)v.y;
     }
     const int m = (int)(idx / N);
-    const double l = lam[m % nch];
-    out[idx] = cx_t<T>{(T)(re * l), (T)(im * l)};
+    const double l = lam ? lam[m % nch] : 1.0;          // no lambda: a plain overlap
+    out[(int64_t)m * ldo + (idx - (int64_t)m * N)] = cx_t<T>{(T)(re * l), (T)(im * l)};
 }
 
 // ---------------------------------------------------------------------------
@@ -692,7 +692,7 @@ static hipError_t reduce_scale(Ctx& c, int nseg, cx_t<T>* partial, int nsplit, i
     const int64_t tot = (int64_t)Mtot * nb;
     const int per = (int)((tot + 255) / 256);
     hipLaunchKernelGGL((k_reduce_scale<T>), dim3((unsigned)(per * nseg)), dim3(256), 0, c.stream, bt, per,
-                       nsplit, Mtot, nb, static_cast<const double*>(c.lam64.ptr), c.nch());
+                       nsplit, Mtot, nb, static_cast<const double*>(c.lam64.ptr), c.nch(), (int64_t)nb);
     return hipGetLastError();
 }
 
@@ -813,6 +813,62 @@ hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accu
     timer_end(c, "proj_apply");
     return e;
 }
+
+// C[m][n] = sum_g conj(A[m][g]) B[n][g] (no lambda), split-K partial sums
+// reduced in FP64: the overlap matrices of an eigensolver's Rayleigh-Ritz
+// step, whose K = npw is long and M, N are band counts.
+// Split count of an overlap: about 512 workgroups, partial sums capped at
+// 32 MiB (an eigensolver's context runs under a small HBM budget).
+template <typename T>
+static int overlap_split(Ctx& c, int m, int n, int k) {
+    const int ns = choose_split(512, gemm_proj_slots<T>(c), m, n, k, 1);
+    const size_t per = (size_t)m * n * sizeof(cx_t<T>);
+    const int cap = (int)std::max<size_t>(1, (size_t(32) << 20) / std::max<size_t>(per, 1));
+    return std::max(1, std::min(ns, cap));
+}
+
+template <typename T>
+size_t overlap_partial_bytes(Ctx& c, int m, int n, int k) {
+    return (size_t)overlap_split<T>(c, m, n, k) * m * n * sizeof(cx_t<T>);
+}
+
+template <typename T>
+hipError_t launch_overlap(Ctx& c, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
+                          void* C, int64_t ldc, void* partial, size_t partial_bytes) {
+    using Cx = cx_t<T>;
+    const int nsplit = overlap_split<T>(c, m, n, k);
+    if ((size_t)nsplit * m * n * sizeof(Cx) > partial_bytes) return hipErrorInvalidValue;
+    int kchunk = (k + nsplit - 1) / nsplit;
+    kchunk = (kchunk + 15) / 16 * 16;
+    ProjBatch<T> bt{};
+    bt.nseg = 1;
+    bt.s[0] = ProjSeg<T>{static_cast<const Cx*>(A), static_cast<const Cx*>(B), static_cast<Cx*>(partial), lda, ldb,
+                         k, kchunk};
+    timer_begin(c);
+    dim3 grid(((m + 63) / 64) * ((n + 63) / 64) * nsplit);
+    hipLaunchKernelGGL((k_gemm_proj<T>), grid, dim3(256), 0, c.stream, bt, m, n, nsplit, m, 0);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        ReduceBatch<T> rb{};
+        rb.nseg = 1;
+        rb.part[0] = static_cast<const Cx*>(partial);
+        rb.out[0] = static_cast<Cx*>(C);
+        const int64_t tot = (int64_t)m * n;
+        const int per = (int)((tot + 255) / 256);
+        hipLaunchKernelGGL((k_reduce_scale<T>), dim3((unsigned)per), dim3(256), 0, c.stream, rb, per, nsplit, m, n,
+                           static_cast<const double*>(nullptr), 1, ldc);
+        e = hipGetLastError();
+    }
+    timer_end(c, "overlap");
+    return e;
+}
+
+template size_t overlap_partial_bytes<double>(Ctx&, int, int, int);
+template size_t overlap_partial_bytes<float>(Ctx&, int, int, int);
+template hipError_t launch_overlap<double>(Ctx&, int, int, int, const void*, int64_t, const void*, int64_t, void*,
+                                           int64_t, void*, size_t);
+template hipError_t launch_overlap<float>(Ctx&, int, int, int, const void*, int64_t, const void*, int64_t, void*,
+                                          int64_t, void*, size_t);
 
 template hipError_t launch_vnl<double>(Ctx&, const ApplySeg*, int, int, void*, size_t, void*);
 template hipError_t launch_vnl<float>(Ctx&, const ApplySeg*, int, int, void*, size_t, void*);

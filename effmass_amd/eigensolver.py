@@ -31,11 +31,11 @@ def _gram(A, B):
     return A.conj() @ B.T
 
 
-def _chol_orth(V, HV=None, tries=3):
+def _chol_orth(V, HV=None, tries=3, gram=None):
     """Orthonormalise the rows of V by Cholesky of the FP64 Gram matrix; HV
     (if given) is transformed alike.  A growing diagonal shift handles a
     near-singular basis."""
-    G = _gram(V, V)
+    G = (gram or _gram)(V, V)
     G = 0.5 * (G + G.conj().T)
     eye = torch.eye(G.shape[0], dtype=G.dtype, device=G.device)
     shift = 0.0
@@ -62,11 +62,36 @@ def tpa_precond(kin, E):
     return p / (p + 16.0 * x ** 4)
 
 
+def _rr(A, B):
+    """Generalized Rayleigh-Ritz A c = lambda B c (FP64, Hermitian, B > 0)
+    through the Cholesky factor of B; returns (lambda ascending, C) with the
+    eigenvectors in the columns of C, or None if B is not positive definite."""
+    A = 0.5 * (A + A.conj().T)
+    B = 0.5 * (B + B.conj().T)
+    L, info = torch.linalg.cholesky_ex(B)
+    if int(info) != 0:
+        return None
+    eye = torch.eye(B.shape[0], dtype=B.dtype, device=B.device)
+    Li = torch.linalg.solve_triangular(L, eye, upper=False)
+    Ap = Li @ A @ Li.conj().T
+    lam, V = torch.linalg.eigh(0.5 * (Ap + Ap.conj().T))
+    return lam, Li.conj().T @ V
+
+
 def lobpcg(apply, X0: torch.Tensor, kin: torch.Tensor, nconv: int | None = None, tol: float = 1e-8,
-           maxiter: int = 300, work_dtype=None) -> EigResult:
+           maxiter: int = 300, work_dtype=None, gram=None) -> EigResult:
     """Lowest nb = X0.shape[0] eigenpairs.  apply(X, HX) writes H X into HX
     (X, HX in work_dtype, default X0's dtype).  Converged when the first
-    nconv (default all) residual norms are below tol."""
+    nconv (default all) residual norms are below tol.  gram(A, B) computes
+    the FP64 overlaps conj(A) @ B.T (default torch; the engine's split-K
+    kernel, Hamiltonian.overlap, is much faster for long rows).
+
+    Each iteration applies H only to the new directions W (residuals of the
+    bands not yet converged, preconditioned and orthonormalised against X);
+    H X and H P are carried along.  The Rayleigh-Ritz step works on the
+    block Gram matrices of [X, W, P] in FP64 (S^H H S and S^H S), solved as
+    a generalized problem through a Cholesky factor; if that factor fails
+    the conjugate block P is dropped for the iteration."""
     dev = X0.device
     cdt = torch.complex128
     wdt = work_dtype or X0.dtype
@@ -74,55 +99,194 @@ def lobpcg(apply, X0: torch.Tensor, kin: torch.Tensor, nconv: int | None = None,
     nconv = nb if nconv is None else nconv
     kin = kin.to(device=dev, dtype=torch.float64)
 
+    gr = gram or _gram
+
     def H(V):
         Vw = V.to(wdt).contiguous()
         HV = torch.empty_like(Vw)
         apply(Vw, HV)
         return HV.to(cdt)
 
-    X, _ = _chol_orth(X0.to(cdt))
+    X, _ = _chol_orth(X0.to(cdt), gram=gram)
     HX = H(X)
-    Hs = _gram(X, HX)
-    e, C = torch.linalg.eigh(0.5 * (Hs + Hs.conj().T))
+    G0 = gr(X, HX)
+    e, C = torch.linalg.eigh(0.5 * (G0 + G0.conj().T))
     X = C.T @ X
     HX = C.T @ HX
     P = HP = None
     it = 0
-    conv = 0
     for it in range(1, maxiter + 1):
         R = HX - e[:, None] * X
         rn = torch.linalg.norm(R, dim=1)
         conv = int((rn[:nconv] < tol).sum())
         if conv == nconv:
             break
-        W = R * tpa_precond(kin, e)
-        # block of the unconverged directions only; W orthogonal to X
-        W = W - (_gram(X, W)).T @ X
-        blocks = [X, W] + ([P] if P is not None else [])
-        S = torch.cat(blocks, 0)
-        S, _ = _chol_orth(S)
-        HS = H(S)
-        Hs = _gram(S, HS)
-        ev, Cv = torch.linalg.eigh(0.5 * (Hs + Hs.conj().T))
+        act = torch.nonzero(rn >= tol).flatten()          # soft locking
+        W = R[act] * tpa_precond(kin, e[act])
+        W = W - gr(X, W).T @ X
+        try:
+            W, _ = _chol_orth(W, gram=gram)
+        except RuntimeError:
+            break                                          # no new directions left
+        HW = H(W)
+        blocks = [(X, HX), (W, HW)] + ([(P, HP)] if P is not None else [])
+        for attempt in range(2):
+            n_blk = len(blocks)
+            A = [[None] * n_blk for _ in range(n_blk)]
+            B = [[None] * n_blk for _ in range(n_blk)]
+            for i in range(n_blk):
+                for j in range(i, n_blk):
+                    Si, HSi = blocks[i]
+                    Sj, HSj = blocks[j]
+                    # X^H X and X^H H X are computed too (not taken as I and
+                    # diag(e)), so rounding drift of X never accumulates
+                    A[i][j] = gr(Si, HSj)
+                    B[i][j] = gr(Si, Sj)
+                    if i != j:
+                        A[j][i] = A[i][j].conj().T
+                        B[j][i] = B[i][j].conj().T
+            sol = _rr(torch.cat([torch.cat(r, 1) for r in A], 0), torch.cat([torch.cat(r, 1) for r in B], 0))
+            if sol is not None or len(blocks) == 2:
+                break
+            blocks = blocks[:2]                            # drop P and retry
+        if sol is None:
+            break
+        lam, Cv = sol
         Cn = Cv[:, :nb]
-        Xn = Cn.T @ S
-        HXn = Cn.T @ HS
+        sizes = [b[0].shape[0] for b in blocks]
+        offs = [0] + list(torch.tensor(sizes).cumsum(0).tolist())
+        Xn = sum(Cn[offs[i]:offs[i + 1]].T @ blocks[i][0] for i in range(len(blocks)))
+        HXn = sum(Cn[offs[i]:offs[i + 1]].T @ blocks[i][1] for i in range(len(blocks)))
         # conjugate direction: the part of the new X outside the old X
-        Cp = Cn.clone()
-        Cp[:nb] = 0
-        P = Cp.T @ S
-        HP = Cp.T @ HS
-        P, HP = _chol_orth(P, HP) if torch.linalg.norm(P) > 0 else (None, None)
-        X, HX, e = Xn, HXn, ev[:nb]
+        P = sum(Cn[offs[i]:offs[i + 1]].T @ blocks[i][0] for i in range(1, len(blocks)))
+        HP = sum(Cn[offs[i]:offs[i + 1]].T @ blocks[i][1] for i in range(1, len(blocks)))
+        try:
+            P, HP = _chol_orth(P, HP, gram=gram)
+        except RuntimeError:
+            P = HP = None
+        X, HX, e = Xn, HXn, lam[:nb]
+    R = HX - e[:, None] * X
+    rn = torch.linalg.norm(R, dim=1)
+    return EigResult(E=e, X=X, HX=HX, resid=rn, iters=it, converged=int((rn[:nconv] < tol).sum()))
+
+
+def _spectral_upper_bound(H, n, dev, steps=12, seed=7):
+    """Upper bound of the spectrum of H (Hermitian): a few Lanczos steps on
+    one random vector; largest Ritz value + the last off-diagonal."""
+    g = torch.Generator(device=dev).manual_seed(seed)
+    v = torch.complex(torch.randn(1, n, generator=g, dtype=torch.float64, device=dev),
+                      torch.randn(1, n, generator=g, dtype=torch.float64, device=dev))
+    v = v / torch.linalg.norm(v)
+    v_prev = torch.zeros_like(v)
+    beta = 0.0
+    alphas, betas = [], []
+    for _ in range(steps):
+        w = H(v)
+        alpha = float(torch.real((v.conj() * w).sum()))
+        w = w - alpha * v - beta * v_prev
+        beta_new = float(torch.linalg.norm(w))
+        alphas.append(alpha)
+        betas.append(beta_new)
+        if beta_new < 1e-12:
+            break
+        v_prev, v, beta = v, w / beta_new, beta_new
+    T = torch.diag(torch.tensor(alphas, dtype=torch.float64))
+    for i in range(len(alphas) - 1):
+        T[i, i + 1] = T[i + 1, i] = betas[i]
+    return float(torch.linalg.eigvalsh(T).max()) + betas[-1]
+
+
+def eigh_host(A: torch.Tensor):
+    """Hermitian eigendecomposition of a small matrix on the host (LAPACK
+    zheevd through numpy), results back on A's device.  For the nb x nb
+    Rayleigh-Ritz problems a GPU eigensolver is a long chain of tiny
+    sequential kernels; on the host it leaves the device to the other
+    queries."""
+    import numpy as np
+    a = A.detach().cpu().numpy()
+    w, v = np.linalg.eigh(0.5 * (a + a.conj().T))
+    return (torch.from_numpy(w).to(A.device), torch.from_numpy(np.ascontiguousarray(v)).to(A.device))
+
+
+def chfsi(apply, X0: torch.Tensor, kin: torch.Tensor, nconv: int | None = None, tol: float = 1e-8,
+          maxiter: int = 60, work_dtype=None, gram=None, degree: int = 12, eigh=None) -> EigResult:
+    """Chebyshev-filtered subspace iteration (Zhou, Saad, Tiago and
+    Chelikowsky, J. Comput. Phys. 219 (2006) 172): each outer step filters
+    the block with a degree-`degree` Chebyshev polynomial that damps
+    [E_top, E_max] (scaled three-term recurrence, `degree` H applies),
+    orthonormalises it (Cholesky of the FP64 overlap) and does one
+    Rayleigh-Ritz of size nb.  Compared with LOBPCG it trades H applies,
+    which the engine makes cheap, for far fewer and smaller dense
+    eigenproblems.  Same interface and result as lobpcg(); rows of X0 beyond
+    nconv act as a buffer that speeds up convergence of the wanted ones.
+    The filter recurrence runs in the work precision (complex64 in the
+    mixed path: it only has to damp the unwanted part of the spectrum); the
+    orthonormalisation overlaps, the Rayleigh-Ritz matrices and the Ritz
+    vectors are FP64.  eigh(A) solves the small Rayleigh-Ritz problems
+    (default torch.linalg.eigh; eigh_host runs them on the host)."""
+    dev = X0.device
+    cdt = torch.complex128
+    wdt = work_dtype or X0.dtype
+    nb = X0.shape[0]
+    nconv = nb if nconv is None else nconv
+    gr = gram or _gram
+    eig = eigh or (lambda A: torch.linalg.eigh(0.5 * (A + A.conj().T)))
+
+    def Hw(V):                               # work precision in and out
+        HV = torch.empty_like(V)
+        apply(V, HV)
+        return HV
+
+    def H(V):
+        return Hw(V.to(wdt).contiguous()).to(cdt)
+
+    b_up = _spectral_upper_bound(H, X0.shape[1], dev)
+    X, _ = _chol_orth(X0.to(cdt), gram=gram)
+    HX = H(X)
+    e, C = eig(gr(X, HX))
+    X = C.T @ X
+    HX = C.T @ HX
+    it = 0
+    for it in range(1, maxiter + 1):
+        R = HX - e[:, None] * X
+        rn = torch.linalg.norm(R, dim=1)
+        if int((rn[:nconv] < tol).sum()) == nconv:
+            break
+        a, a0 = float(e[-1]), float(e[0])
+        b = max(b_up, a + 1e-3 * max(1.0, abs(a)))
+        half, mid = 0.5 * (b - a), 0.5 * (b + a)
+        sigma = half / (a0 - mid)
+        tau = 2.0 / sigma
+        # Y1 = (H - mid) X sigma / half, from the FP64 HX of the last step
+        Y = ((HX - mid * X) * (sigma / half)).to(wdt)
+        Xp = X.to(wdt)
+        for _ in range(2, degree + 1):
+            sig_new = 1.0 / (tau - sigma)
+            # Y_next = (H - mid) Y 2 sig_new / half - sigma sig_new Y_prev, in place
+            Yn = Hw(Y)
+            Yn.mul_(2.0 * sig_new / half).add_(Y, alpha=-mid * 2.0 * sig_new / half).add_(Xp, alpha=-sigma * sig_new)
+            Xp, Y, sigma = Y, Yn, sig_new
+        Y = Y.to(cdt)
+        try:
+            X, _ = _chol_orth(Y, gram=gram)
+        except RuntimeError:
+            X, _ = _chol_orth(torch.linalg.qr(Y.T)[0].T.contiguous(), gram=gram)
+        HX = H(X)
+        e, C = eig(gr(X, HX))
+        X = C.T @ X
+        HX = C.T @ HX
     R = HX - e[:, None] * X
     rn = torch.linalg.norm(R, dim=1)
     return EigResult(E=e, X=X, HX=HX, resid=rn, iters=it, converged=int((rn[:nconv] < tol).sum()))
 
 
 def random_start(nb, npw, kin, seed=0, device="cpu"):
-    """Deterministic start vectors: complex normal weighted by 1/(1+T(G))."""
-    g = torch.Generator(device="cpu").manual_seed(seed)
-    re = torch.randn(nb, npw, generator=g, dtype=torch.float64)
-    im = torch.randn(nb, npw, generator=g, dtype=torch.float64)
-    w = 1.0 / (1.0 + kin.detach().cpu().to(torch.float64))
-    return ((re + 1j * im) * w[None, :]).to(device)
+    """Deterministic start vectors: complex normal weighted by 1/(1+T(G)),
+    drawn on `device` (the same seed gives the same vectors on the same
+    kind of device)."""
+    dev = torch.device(device)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    re = torch.randn(nb, npw, generator=g, dtype=torch.float64, device=dev)
+    im = torch.randn(nb, npw, generator=g, dtype=torch.float64, device=dev)
+    w = 1.0 / (1.0 + kin.detach().to(device=dev, dtype=torch.float64))
+    return torch.complex(re, im) * w[None, :]

@@ -238,6 +238,26 @@ class Hamiltonian:
                    self._ctx, "effm_dhdk_multi")
         return outs
 
+    def overlap(self, A, B):
+        """C = conj(A) @ B.T for device tensors A [m, k], B [n, k] (complex128
+        -> FP64 path, complex64 -> mixed), split-K sums added in FP64; C has
+        A's dtype."""
+        import torch
+        if A.dtype != B.dtype or A.dtype not in (torch.complex128, torch.complex64):
+            raise TypeError("overlap takes two complex128 or two complex64 tensors")
+        A = A.contiguous()
+        B = B.contiguous()
+        m, k = A.shape
+        n = B.shape[0]
+        if B.shape[1] != k:
+            raise ValueError("A and B need the same number of columns")
+        Cm = torch.empty((m, n), dtype=A.dtype, device=A.device)
+        p = _lib.EFFM_PREC_FP64 if A.dtype == torch.complex128 else _lib.EFFM_PREC_MIXED
+        self._sync_stream()
+        _lib.check(self._lib.effm_overlap(self._ctx, m, n, k, A.data_ptr(), max(k, 1), B.data_ptr(), max(k, 1),
+                                          Cm.data_ptr(), max(n, 1), p, 0), self._ctx, "effm_overlap")
+        return Cm
+
     def autotune(self, ikpt: int, n_bands: int, prec: str = "fp64"):
         """Time the candidate launch shapes at this k-point and keep the
         fastest for `prec`; returns (bands per pencil workgroup, split-K

@@ -155,6 +155,16 @@ int effm_dhdk_multi(effm_ctx* ctx, int nk, const int* ikpts, int nb,
 int effm_rayleigh(effm_ctx* ctx, int npw, int nb, const void* psi, int64_t ld,
                   const void* hpsi, int64_t ld_h, int prec, double* E, int flags);
 
+/* Overlap matrix C[i][j] = sum_g conj(A[i][g]) B[j][g] for i < m, j < n
+ * over k coefficients (device pointers, row-major, leading dimensions in
+ * complex elements): complex128 in and out for EFFM_PREC_FP64, complex64
+ * for EFFM_PREC_MIXED; split-K partial sums are added in FP64 either way.
+ * The overlaps / subspace Hamiltonians of an eigensolver's Rayleigh-Ritz
+ * step.  No reference counterpart (the reference has no eigensolver,
+ * docs/roadmap.md:5-7). */
+int effm_overlap(effm_ctx* ctx, int m, int n, int k, const void* A, int64_t lda,
+                 const void* B, int64_t ldb, void* C, int64_t ldc, int prec, int flags);
+
 /* Autotuner: times the candidate launch shapes (bands per x-pencil
  * workgroup: 4 / 8 / 16; split-K workgroup target of the projection GEMM:
  * 512 ... 2048) with nb bands at k-point ikpt in precision prec, on
