@@ -14,9 +14,24 @@ namespace effm {
 // Radix pair (first DIT stage P1, second P2) of the specialised axes.  Must
 // match factor_axis() on the host for these sizes.
 template <int N> struct Plan2 { static constexpr bool ok = false; static constexpr int P1 = 1, P2 = 1; };
-template <> struct Plan2<90> { static constexpr bool ok = true; static constexpr int P1 = 10, P2 = 9; };
-template <> struct Plan2<48> { static constexpr bool ok = true; static constexpr int P1 = 8, P2 = 6; };
-template <> struct Plan2<36> { static constexpr bool ok = true; static constexpr int P1 = 6, P2 = 6; };
+#define EFFM_PLAN2(N, A, B) \
+    template <> struct Plan2<N> { static constexpr bool ok = true; static constexpr int P1 = A, P2 = B; };
+EFFM_PLAN2(15, 5, 3)
+EFFM_PLAN2(16, 4, 4)
+EFFM_PLAN2(18, 6, 3)
+EFFM_PLAN2(20, 5, 4)
+EFFM_PLAN2(24, 6, 4)
+EFFM_PLAN2(25, 5, 5)
+EFFM_PLAN2(36, 6, 6)
+EFFM_PLAN2(40, 8, 5)
+EFFM_PLAN2(45, 9, 5)
+EFFM_PLAN2(48, 8, 6)
+EFFM_PLAN2(90, 10, 9)
+#undef EFFM_PLAN2
+
+// The axis lengths above (the 64-atom cell's 90, the 8-atom 48, and the
+// grids of 2-64 atom cells at 30-50 Ry); X-macro for dispatch.
+#define EFFM_CT_SIZES(X) X(15) X(16) X(18) X(20) X(24) X(25) X(36) X(40) X(45) X(48) X(90)
 
 // One radix-P stage (combining blocks of L) over NL lines of an N-point axis;
 // element e of line l at l*LS + e*ES.  Lines vary fastest across threads when

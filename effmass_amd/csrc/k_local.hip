@@ -486,9 +486,10 @@ template <> struct PencilShape<double, 90> { static constexpr int TZ = 18; };
 
 static int Plan2N(int n, int which) {
     switch (n) {
-        case 90: return which == 1 ? Plan2<90>::P1 : Plan2<90>::P2;
-        case 48: return which == 1 ? Plan2<48>::P1 : Plan2<48>::P2;
-        case 36: return which == 1 ? Plan2<36>::P1 : Plan2<36>::P2;
+#define EFFM_PLAN_CASE(N) \
+    case N: return which == 1 ? Plan2<N>::P1 : Plan2<N>::P2;
+        EFFM_CT_SIZES(EFFM_PLAN_CASE)
+#undef EFFM_PLAN_CASE
         default: return -1;
     }
 }
@@ -560,6 +561,29 @@ static void launch_pencil_ct(Ctx& c, const LocalBatch<T>& bt, const T* vl, const
 }
 
 template <typename T>
+static hipError_t planes_ct(Ctx& c, int n, const LocalBatch<T>& bt, int nitems, bool accumulate, bool g2r,
+                            const cx_t<T>* tw) {
+    switch (n) {
+#define EFFM_PLANES_CASE(N) \
+    case N: return launch_planes_ct<T, N>(c, bt, nitems, accumulate, g2r, tw);
+        EFFM_CT_SIZES(EFFM_PLANES_CASE)
+#undef EFFM_PLANES_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+static void pencil_ct(Ctx& c, int n, const LocalBatch<T>& bt, const T* vl, const cx_t<T>* tw1) {
+    switch (n) {
+#define EFFM_PENCIL_CASE(N) \
+    case N: launch_pencil_ct<T, N>(c, bt, vl, tw1); break;
+        EFFM_CT_SIZES(EFFM_PENCIL_CASE)
+#undef EFFM_PENCIL_CASE
+        default: break;
+    }
+}
+
+template <typename T>
 static size_t plane_lds_bytes(const Ctx& c) {
     return sizeof(cx_t<T>) * ((size_t)c.n2 * c.n3 + c.n2 + c.n3);
 }
@@ -615,18 +639,11 @@ hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, 
     const size_t lds_pencil = sizeof(C) * ((size_t)c.n1 * TZ + c.n1);
     hipError_t e;
 
-    // compile-time plans for the common square planes / pencil lengths
-    const int np = c.n2;
-    const bool sq = c.n2 == c.n3;
-    auto ct_plane = [&](int n) {
-        return sq && np == n && plan_is(c.ax2, Plan2N(n, 1), Plan2N(n, 2)) &&
-               plan_is(c.ax3, Plan2N(n, 1), Plan2N(n, 2));
-    };
-    auto ct_pencil = [&](int n) { return c.n1 == n && plan_is(c.ax1, Plan2N(n, 1), Plan2N(n, 2)); };
-    const bool pl90 = ct_plane(90), pl48 = ct_plane(48), pl36 = ct_plane(36);
-    const bool pe90 = ct_pencil(90), pe48 = ct_pencil(48), pe36 = ct_pencil(36);
-    const bool plct = pl90 || pl48 || pl36;
-    if (!plct || !(pe90 || pe48 || pe36)) {
+    // compile-time plans for square planes / pencil lengths of EFFM_CT_SIZES
+    const bool plct = c.n2 == c.n3 && Plan2N(c.n2, 1) > 0 && plan_is(c.ax2, Plan2N(c.n2, 1), Plan2N(c.n2, 2)) &&
+                      plan_is(c.ax3, Plan2N(c.n3, 1), Plan2N(c.n3, 2));
+    const bool pect = Plan2N(c.n1, 1) > 0 && plan_is(c.ax1, Plan2N(c.n1, 1), Plan2N(c.n1, 2));
+    if (!plct || !pect) {
         // generic kernels: dynamic LDS sized by this cell's grid
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_g2r<T>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
@@ -640,9 +657,7 @@ hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, 
     }
     const dim3 gp(maxplanes, nb, nseg);
     timer_begin(c);
-    if (pl90) e = launch_planes_ct<T, 90>(c, bt, nitems, accumulate, true, tw2);
-    else if (pl48) e = launch_planes_ct<T, 48>(c, bt, nitems, accumulate, true, tw2);
-    else if (pl36) e = launch_planes_ct<T, 36>(c, bt, nitems, accumulate, true, tw2);
+    if (plct) e = planes_ct<T>(c, c.n2, bt, nitems, accumulate, true, tw2);
     else {
         hipLaunchKernelGGL((k_plane_g2r<T>), gp, dim3(512), lds_plane, c.stream, bt, c.ax2, c.ax3, tw2, tw3);
         e = hipGetLastError();
@@ -650,9 +665,7 @@ hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, 
     timer_end(c, "plane_g2r");
     if (e != hipSuccess) return e;
     timer_begin(c);
-    if (pe90) launch_pencil_ct<T, 90>(c, bt, vl, tw1);
-    else if (pe48) launch_pencil_ct<T, 48>(c, bt, vl, tw1);
-    else if (pe36) launch_pencil_ct<T, 36>(c, bt, vl, tw1);
+    if (pect) pencil_ct<T>(c, c.n1, bt, vl, tw1);
     else {
         const int nzc = (c.n3 + TZ - 1) / TZ;
         hipLaunchKernelGGL((k_xpencil_vloc<T, TZ>), dim3(c.n2 * nzc, nb, nseg), dim3(256), lds_pencil, c.stream,
@@ -661,9 +674,7 @@ hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, 
     timer_end(c, "xpencil_vloc");
     if ((e = hipGetLastError()) != hipSuccess) return e;
     timer_begin(c);
-    if (pl90) e = launch_planes_ct<T, 90>(c, bt, nitems, accumulate, false, tw2);
-    else if (pl48) e = launch_planes_ct<T, 48>(c, bt, nitems, accumulate, false, tw2);
-    else if (pl36) e = launch_planes_ct<T, 36>(c, bt, nitems, accumulate, false, tw2);
+    if (plct) e = planes_ct<T>(c, c.n2, bt, nitems, accumulate, false, tw2);
     else {
         T scale = (T)(1.0 / ((double)c.n1 * c.n2 * c.n3));
         hipLaunchKernelGGL((k_plane_r2g<T>), gp, dim3(512), lds_plane, c.stream, bt, scale, accumulate ? 1 : 0,

@@ -281,9 +281,13 @@ def test_multi_k_ragged():
     H.close()
 
 
-@pytest.mark.parametrize("grid", [(28, 30, 32), (42, 45, 40), (24, 20, 36), (48, 49, 50)])
+@pytest.mark.parametrize("grid", [(28, 30, 32), (42, 45, 40), (24, 20, 36), (48, 49, 50),
+                                  (15, 15, 15), (16, 16, 16), (18, 18, 18), (20, 20, 20), (25, 25, 25),
+                                  (40, 24, 24), (45, 45, 45), (48, 25, 25)])
 def test_fft_radix_plans(grid):
-    """Grids whose axes use radices 2..12 (incl. 7) in the LDS plan."""
+    """Grids whose axes use radices 2..12 (incl. 7) in the runtime LDS plan,
+    and every compile-time two-stage plan (square planes and pencils of
+    15, 16, 18, 20, 24, 25, 40, 45, 48)."""
     wl = synth.custom(*synth.fcc_primitive(10.26), grid, 12.0,
                       [np.array([0.13, 0.0, -0.27])], 4, seed=sum(grid))
     got = run_h(wl)
