@@ -126,8 +126,10 @@ def build_workload(q: MassQuery):
 
 def estimate_bytes(q: MassQuery, wl=None, prec="mixed"):
     """Device bytes a query holds while it runs: engine data (V_loc, tables,
-    projectors and their k-gradients, FFT workspace for a 3nb LOBPCG block,
-    split-K partials) plus the solver's blocks (FP64 X, HX, S, HS, P)."""
+    projectors and their k-gradients at the k-point and at the shifted
+    k-point of the effective-mass step, FFT workspace for a 3nb block,
+    split-K and overlap partials) plus the solver's blocks (FP64 X, HX and
+    the filter / search directions)."""
     if wl is not None:
         c = wl.cell
         A, n, nch = c.A, c.n, c.nchannels()
@@ -144,7 +146,9 @@ def estimate_bytes(q: MassQuery, wl=None, prec="mixed"):
     S = 8 if prec == "mixed" else 16
     blk = 3 * q.nb
     planes = min(n[0], int(2 * math.sqrt(q.ecut_ry) * np.linalg.norm(A[0]) / (2 * math.pi)) + 2)
-    engine = N * 12 + 4 * nch * npw * S + planes * n[1] * n[2] * S * blk + 32 * nch * blk * S
+    # projectors P and their k-gradients dP (4 nch x npw) for the k-point and
+    # for the shifted copy the effective-mass step loads next to it
+    engine = N * 12 + 2 * 4 * nch * npw * S + planes * n[1] * n[2] * S * blk + 32 * nch * blk * S
     solver = 16 * npw * (6 * blk + 4 * q.nb) + 16 * blk * blk * 4
     return int(1.25 * (engine + solver)) + (64 << 20) + (32 << 20)   # + overlap partial sums
 

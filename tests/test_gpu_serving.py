@@ -44,3 +44,12 @@ def test_admission_by_hbm_footprint():
     ans2, stats2 = serving.serve([qs[0], big], slots=2, realtime=False, hbm_budget=need)
     assert ans2[9].error and "budget" in ans2[9].error
     assert ans2[0].error is None
+
+
+def test_footprint_estimate_covers_a_large_cell_with_few_bands():
+    """The largest projector share per band: 64 atoms at 50 Ry with 16
+    bands (projectors and their k-gradients at k and at the shifted k of
+    the effective-mass step) runs within its own estimate."""
+    q = serving.MassQuery(qid=0, natom=64, ecut_ry=50.0, nb=16, kfracs=[np.array([0.1, -0.2, 0.3])], seed=3)
+    a = serving.run_query(q)                  # budget = estimate_bytes(q)
+    assert a.error is None and len(a.inv_mass) == 1
