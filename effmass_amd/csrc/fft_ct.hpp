@@ -120,6 +120,83 @@ __device__ __forceinline__ void ct_fft_rows(cx_t<T>* buf, const cx_t<T>* tw, con
     }
 }
 
+// The radix-P2 stage of the column (strided) transform of an N x N plane
+// with global memory on one side: the last DIT stage writing its results
+// straight to the rows of a global plane (row pitch WP >= N), or the first
+// DIF stage taking its inputs from one.  With N = P1 * P2 that stage has P1
+// butterflies per column; work item t takes column t % W (consecutive lanes
+// on consecutive columns, so the global accesses of one instruction are row
+// segments) and butterfly r = t / W, whose elements are rows r + u * P1.
+// Each skips an LDS round trip and a barrier against ct_fft + a copy.
+template <int N, int W, int BLOCK>
+struct ColStage {
+    static constexpr int P1 = Plan2<N>::P1, P2 = Plan2<N>::P2;
+    static constexpr int TOTAL = W * P1;
+    static constexpr int ITER = (TOTAL + BLOCK - 1) / BLOCK;
+};
+
+// G2R: last DIT stage of the columns, from LDS to out[row * WP + col].
+// The pad columns N <= col < WP repeat column N - 1 (whole rows are written).
+template <int N, int WP, int S, int BLOCK, typename T>
+__device__ __forceinline__ void ct_cols_last_to_global(const cx_t<T>* buf, const cx_t<T>* tw, cx_t<T>* out) {
+    using Cs = ColStage<N, WP, BLOCK>;
+#pragma unroll
+    for (int it = 0; it < Cs::ITER; ++it) {
+        const int t = threadIdx.x + it * BLOCK;
+        if (Cs::ITER * BLOCK == Cs::TOTAL || t < Cs::TOTAL) {
+            const int r = t / WP, col = t - r * WP;
+            const int c = min(col, N - 1);
+            cx_t<T> y[Cs::P2];
+#pragma unroll
+            for (int u = 0; u < Cs::P2; ++u) y[u] = buf[(r + u * Cs::P1) * N + c];
+            if (r != 0) {
+#pragma unroll
+                for (int u = 1; u < Cs::P2; ++u) y[u] = cmul(y[u], twid<S>(tw, u * r));
+            }
+            dft<Cs::P2, S, T>(y);
+#pragma unroll
+            for (int u = 0; u < Cs::P2; ++u) out[(int64_t)(r + u * Cs::P1) * WP + col] = y[u];
+        }
+    }
+}
+
+// R2G: the inputs of this thread's first DIF column butterflies, from
+// in[row * WP + col], col < N.  Spare work items load a valid duplicate.
+template <int N, int WP, int BLOCK, typename T>
+__device__ __forceinline__ void ct_cols_first_load(const cx_t<T>* in,
+                                                   cx_t<T> (&y)[ColStage<N, N, BLOCK>::ITER][Plan2<N>::P2]) {
+    using Cs = ColStage<N, N, BLOCK>;
+#pragma unroll
+    for (int it = 0; it < Cs::ITER; ++it) {
+        const int t = min((int)threadIdx.x + it * BLOCK, Cs::TOTAL - 1);
+        const int r = t / N, col = t - r * N;
+#pragma unroll
+        for (int u = 0; u < Cs::P2; ++u) y[it][u] = in[(int64_t)(r + u * Cs::P1) * WP + col];
+    }
+}
+
+// R2G: first DIF stage of the columns on the loaded inputs, results to the
+// LDS plane (no barrier).
+template <int N, int S, int BLOCK, typename T>
+__device__ __forceinline__ void ct_cols_first_to_lds(cx_t<T>* buf, const cx_t<T>* tw,
+                                                     cx_t<T> (&y)[ColStage<N, N, BLOCK>::ITER][Plan2<N>::P2]) {
+    using Cs = ColStage<N, N, BLOCK>;
+#pragma unroll
+    for (int it = 0; it < Cs::ITER; ++it) {
+        const int t = threadIdx.x + it * BLOCK;
+        if (Cs::ITER * BLOCK == Cs::TOTAL || t < Cs::TOTAL) {
+            const int r = t / N, col = t - r * N;
+            dft<Cs::P2, S, T>(y[it]);
+            if (r != 0) {
+#pragma unroll
+                for (int u = 1; u < Cs::P2; ++u) y[it][u] = cmul(y[it][u], twid<S>(tw, u * r));
+            }
+#pragma unroll
+            for (int u = 0; u < Cs::P2; ++u) buf[(r + u * Cs::P1) * N + col] = y[it][u];
+        }
+    }
+}
+
 constexpr int round_up(int a, int b) { return (a + b - 1) / b * b; }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int cmin(int a, int b) { return a < b ? a : b; }

@@ -211,9 +211,7 @@ k_plane_g2r_ct(const LocalBatch<T> bt, int nitems, const cx_t<T>* __restrict__ t
     using C = cx_t<T>;
     constexpr int N2 = N * N;
     constexpr int WP = ws_pitch<T>(N);
-    constexpr int NP = N * WP;                          // rows written (pad columns included)
     constexpr int64_t PS = ws_plane<T>(N, N);           // workspace plane stride
-    constexpr int NOUT = (NP + BLOCK - 1) / BLOCK;
     constexpr int NZ = (N2 + BLOCK - 1) / BLOCK;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* plane = reinterpret_cast<C*>(smem_raw);
@@ -237,16 +235,11 @@ k_plane_g2r_ct(const LocalBatch<T> bt, int nitems, const cx_t<T>* __restrict__ t
         for (int e = e0 + threadIdx.x; e < e1; e += BLOCK) plane[sg.ent_pos[e]] = ps[sg.ent_ig[e]];
         __syncthreads();
         ct_fft_rows<N, +1, true, BLOCK, T>(plane, tw, srows, nrows);   // along k, rows with data
-        ct_fft<N, +1, true, N, 1, N, BLOCK, T>(plane, tw);             // along j (columns)
-        C* o = sg.work + ((int64_t)b * sg.nplanes + p) * PS;
-#pragma unroll
-        for (int it = 0; it < NOUT; ++it) {
-            const int idx = threadIdx.x + it * BLOCK;
-            const int j = idx / WP;
-            const int kk = idx - j * WP;
-            // pad columns of the row (FP32 rows of 128-byte pitch) carry junk
-            if (NOUT * BLOCK == NP || idx < NP) o[idx] = plane[min(j, N - 1) * N + min(kk, N - 1)];
-        }
+        // along j (columns): first stage in LDS, the last one stores the rows
+        // (pad columns of FP32 rows of 128-byte pitch carry junk)
+        ct_stage<N, Plan2<N>::P1, 1, +1, true, N, 1, N, BLOCK, T>(plane, tw);
+        __syncthreads();
+        ct_cols_last_to_global<N, WP, +1, BLOCK, T>(plane, tw, sg.work + ((int64_t)b * sg.nplanes + p) * PS);
         __syncthreads();
     }
 }
@@ -258,54 +251,34 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
     using C = cx_t<T>;
     constexpr int N2 = N * N;
     constexpr int WP = ws_pitch<T>(N);
-    constexpr int NP = N * WP;
     constexpr int64_t PS = ws_plane<T>(N, N);
-    constexpr int NIN = (NP + BLOCK - 1) / BLOCK;
     constexpr int PF = 2;
+    using Cs = ColStage<N, N, BLOCK>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* plane = reinterpret_cast<C*>(smem_raw);
     C* tw = plane + N2;
     int* srows = reinterpret_cast<int*>(tw + N);        // rows of the plane that are gathered
     if (threadIdx.x < N) tw[threadIdx.x] = tw_g[threadIdx.x];
+    __syncthreads();                                    // tw is read before the first item's barrier
     auto plane_in = [&](int it) {
         const LocalSeg<T>& sg = bt.s[seg_of(bt, it)];
         const int local = it - sg.item0;
         const int p = local % sg.nplanes, b = local / sg.nplanes;
         return static_cast<const C*>(sg.work) + ((int64_t)b * sg.nplanes + p) * PS;
     };
-    // FP32: the next plane is prefetched into registers behind this one's
-    // FFT; FP64 loads each plane at the top of its item
-    constexpr int NIV = kPipeR2G<T> ? NIN : 1;
-    C iv[NIV];
-    auto prefetch = [&](int it) {
-        const C* in = plane_in(it);
-#pragma unroll
-        for (int q = 0; q < NIV; ++q) {
-            const int idx = threadIdx.x + q * BLOCK;
-            iv[q] = in[min(idx, NP - 1)];
-        }
-    };
-    auto to_lds = [&](int q, const C& v) {
-        const int idx = threadIdx.x + q * BLOCK;
-        const int j = idx / WP;
-        const int kk = idx - j * WP;
-        if (kk < N && j < N) plane[j * N + kk] = v;
-    };
+    // The first column stage (along j) runs on registers loaded from the
+    // workspace plane.  FP32 loads the next plane's behind this one's FFT;
+    // FP64 loads each plane's at the top of its item.
+    C y[Cs::ITER][Cs::P2];
     int item = blockIdx.x;
     if constexpr (kPipeR2G<T>)
-        if (item < nitems) prefetch(item);
+        if (item < nitems) ct_cols_first_load<N, WP, BLOCK, T>(plane_in(item), y);
     for (; item < nitems; item += gridDim.x) {
         const LocalSeg<T>& sg = bt.s[seg_of(bt, item)];
         const int local = item - sg.item0;
         const int p = local % sg.nplanes, b = local / sg.nplanes;
-        if constexpr (kPipeR2G<T>) {
-#pragma unroll
-            for (int q = 0; q < NIN; ++q) to_lds(q, iv[q]);
-        } else {
-            const C* in = plane_in(item);
-#pragma unroll
-            for (int q = 0; q < NIN; ++q) to_lds(q, in[min((int)threadIdx.x + q * BLOCK, NP - 1)]);
-        }
+        if constexpr (!kPipeR2G<T>) ct_cols_first_load<N, WP, BLOCK, T>(plane_in(item), y);
+        ct_cols_first_to_lds<N, -1, BLOCK, T>(plane, tw, y);
         const int r0 = sg.row_ptr[p], nrows = sg.row_ptr[p + 1] - r0;
         if (threadIdx.x < nrows) srows[threadIdx.x] = sg.rows[r0 + threadIdx.x];
         __syncthreads();
@@ -347,8 +320,9 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
                 }
             }
             const int nxt = item + gridDim.x;
-            if (nxt < nitems) prefetch(nxt);
-            ct_fft<N, -1, false, N, 1, N, BLOCK, T>(plane, tw);          // along j
+            if (nxt < nitems) ct_cols_first_load<N, WP, BLOCK, T>(plane_in(nxt), y);
+            ct_stage<N, Cs::P1, 1, -1, false, N, 1, N, BLOCK, T>(plane, tw);   // along j, last stage
+            __syncthreads();
             ct_fft_rows<N, -1, false, BLOCK, T>(plane, tw, srows, nrows); // along k, gathered rows
 #pragma unroll
             for (int f = 0; f < PF; ++f) {
@@ -360,7 +334,8 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
             }
             for (int e = e0 + threadIdx.x + PF * BLOCK; e < e1; e += BLOCK) gather(e);
         } else {
-            ct_fft<N, -1, false, N, 1, N, BLOCK, T>(plane, tw);          // along j
+            ct_stage<N, Cs::P1, 1, -1, false, N, 1, N, BLOCK, T>(plane, tw);   // along j, last stage
+            __syncthreads();
             ct_fft_rows<N, -1, false, BLOCK, T>(plane, tw, srows, nrows); // along k, gathered rows
             for (int e = e0 + threadIdx.x; e < e1; e += BLOCK) gather(e);
         }
