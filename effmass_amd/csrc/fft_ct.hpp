@@ -160,9 +160,10 @@ __device__ __forceinline__ void ct_cols_last_to_global(const cx_t<T>* buf, const
     }
 }
 
-// R2G: the inputs of this thread's first DIF column butterflies, from
-// in[row * WP + col], col < N.  Spare work items load a valid duplicate.
-template <int N, int WP, int BLOCK, typename T>
+// R2G: the inputs u in [U0, U1) of this thread's first DIF column
+// butterflies, from in[row * WP + col], col < N.  Spare work items load a
+// valid duplicate.
+template <int N, int WP, int BLOCK, typename T, int U0 = 0, int U1 = Plan2<N>::P2>
 __device__ __forceinline__ void ct_cols_first_load(const cx_t<T>* in,
                                                    cx_t<T> (&y)[ColStage<N, N, BLOCK>::ITER][Plan2<N>::P2]) {
     using Cs = ColStage<N, N, BLOCK>;
@@ -171,7 +172,7 @@ __device__ __forceinline__ void ct_cols_first_load(const cx_t<T>* in,
         const int t = min((int)threadIdx.x + it * BLOCK, Cs::TOTAL - 1);
         const int r = t / N, col = t - r * N;
 #pragma unroll
-        for (int u = 0; u < Cs::P2; ++u) y[it][u] = in[(int64_t)(r + u * Cs::P1) * WP + col];
+        for (int u = U0; u < U1; ++u) y[it][u] = in[(int64_t)(r + u * Cs::P1) * WP + col];
     }
 }
 

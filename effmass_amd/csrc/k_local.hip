@@ -197,98 +197,224 @@ k_plane_r2g(const LocalBatch<T> bt, T scale, int accumulate, FftAxis a2, FftAxis
 //  with compile-time two-stage plans (fft_ct.hpp).  Same data flow as above.
 // ---------------------------------------------------------------------------
 // Persistent plane kernels: each workgroup walks the flat (segment, plane,
-// band) items with stride gridDim.x.  The R2G pass in FP32 loads the next
-// item's plane into registers during this item's FFT; in FP64 those
-// registers are taken by the FFT itself.  (Measured on the 90^3 grid: the
-// same overlap in the G2R pass, the next item's scatter loads issued before
-// this item's stores, made that pass 30% slower, so G2R does not pipeline.)
+// band) items with stride gridDim.x.  One FP64 plane fills 129.6 KB of the
+// 160 KB LDS, so a CU holds one workgroup and nothing else hides the memory
+// latency of an item's dependent loads (CSR offsets -> entry indices ->
+// psi).  The kernels load inputs ahead, a level at a time, behind the FFT
+// stages (per precision, PlaneTune):
+//   SPF  G2R: the next item's row list and its first SPF scatter entries per
+//        thread (indices during the row FFT, psi values during the columns);
+//   GPF  R2G: this item's first GPF gather entries per thread (indices
+//        during the first column stage, psi / kin / hpsi after it);
+//   YPF  R2G: column-stage inputs u < YPF of the next plane, during the
+//        gather (the rest at the top of the item);
+//   G2RW / R2GW  waves per SIMD the register allocation must allow (8 lets
+//        two FP32 workgroups of 15 waves share a CU).
+// Entries past SPF / GPF per thread (the densest planes) load directly.
+// Measured on the 90^3 grid (256 bands): FP64 R2G 420 -> 388 us with GPF 1;
+// FP32 R2G 239 -> 228 us at two workgroups per CU, where prefetching costs
+// the registers that allow the second; YPF > 0 spills in FP64 and did not
+// pay in FP32; SPF changes G2R by 1-3%.
 template <typename T>
-constexpr bool kPipeR2G = sizeof(T) == 4;
+struct PlaneTune;
+template <>
+struct PlaneTune<double> {
+    static constexpr int G2RW = 1, R2GW = 1, SPF = 1, GPF = 1, YPF = 0;
+};
+template <>
+struct PlaneTune<float> {
+    static constexpr int G2RW = 1, R2GW = 8, SPF = 2, GPF = 0, YPF = 0;
+};
+
+// Flat item -> (segment, plane, band).
+template <typename T>
+struct PlaneItem {
+    const LocalSeg<T>* sg;
+    int p, b;
+};
+template <typename T>
+__device__ __forceinline__ PlaneItem<T> plane_item(const LocalBatch<T>& bt, int item) {
+    const LocalSeg<T>& sg = bt.s[seg_of(bt, item)];
+    const int local = item - sg.item0;
+    const int p = local % sg.nplanes;
+    return PlaneItem<T>{&sg, p, local / sg.nplanes};
+}
 
 template <typename T, int N, int BLOCK>
-__global__ void __launch_bounds__(BLOCK)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PlaneTune<T>::G2RW)))
 k_plane_g2r_ct(const LocalBatch<T> bt, int nitems, const cx_t<T>* __restrict__ tw_g) {
     using C = cx_t<T>;
     constexpr int N2 = N * N;
     constexpr int WP = ws_pitch<T>(N);
     constexpr int64_t PS = ws_plane<T>(N, N);           // workspace plane stride
     constexpr int NZ = (N2 + BLOCK - 1) / BLOCK;
+    constexpr int SPF = PlaneTune<T>::SPF;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* plane = reinterpret_cast<C*>(smem_raw);
     C* tw = plane + N2;
     int* srows = reinterpret_cast<int*>(tw + N);        // rows of the plane holding data
     if (threadIdx.x < N) tw[threadIdx.x] = tw_g[threadIdx.x];
-    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
-        const LocalSeg<T>& sg = bt.s[seg_of(bt, item)];
-        const int local = item - sg.item0;
-        const int p = local % sg.nplanes, b = local / sg.nplanes;
-        const int r0 = sg.row_ptr[p], nrows = sg.row_ptr[p + 1] - r0;
-        if (threadIdx.x < nrows) srows[threadIdx.x] = sg.rows[r0 + threadIdx.x];
+    // the next item's row count, this thread's row-list entry, scatter
+    // positions (-1: none) and values
+    int nrows = 0, srow = 0, spos[SPF > 0 ? SPF : 1];
+    C sval[SPF > 0 ? SPF : 1];
+    auto fetch_rows = [&](const PlaneItem<T>& q) {
+        const int r0 = q.sg->row_ptr[q.p];
+        nrows = q.sg->row_ptr[q.p + 1] - r0;
+        if (threadIdx.x < nrows) srow = q.sg->rows[r0 + threadIdx.x];
+    };
+    int sig[SPF > 0 ? SPF : 1];
+    auto fetch_index = [&](const PlaneItem<T>& q) {
+        const int e0 = q.sg->plane_ptr[q.p], e1 = q.sg->plane_ptr[q.p + 1];
+#pragma unroll
+        for (int f = 0; f < SPF; ++f) {
+            const int e = e0 + threadIdx.x + f * BLOCK;
+            spos[f] = -1;
+            if (e < e1) {
+                spos[f] = q.sg->ent_pos[e];
+                sig[f] = q.sg->ent_ig[e];
+            }
+        }
+    };
+    auto fetch_values = [&](const PlaneItem<T>& q) {
+        const C* ps = q.sg->psi + (int64_t)q.b * q.sg->ld;
+#pragma unroll
+        for (int f = 0; f < SPF; ++f)
+            if (spos[f] >= 0) sval[f] = ps[sig[f]];
+    };
+    int item = blockIdx.x;
+    if (item < nitems) {
+        const PlaneItem<T> q = plane_item(bt, item);
+        fetch_rows(q);
+        fetch_index(q);
+        fetch_values(q);
+    }
+    for (; item < nitems; item += gridDim.x) {
+        const PlaneItem<T> cur = plane_item(bt, item);
+        const int nxt = item + gridDim.x;
+        const PlaneItem<T> q = plane_item(bt, nxt < nitems ? nxt : item);
+        const int nr = nrows;
+        if (threadIdx.x < nr) srows[threadIdx.x] = srow;
 #pragma unroll
         for (int it = 0; it < NZ; ++it) {
             const int i = threadIdx.x + it * BLOCK;
             if (NZ * BLOCK == N2 || i < N2) plane[i] = C{0, 0};
         }
         __syncthreads();
-        const C* ps = sg.psi + (int64_t)b * sg.ld;
-        const int e0 = sg.plane_ptr[p], e1 = sg.plane_ptr[p + 1];
-        for (int e = e0 + threadIdx.x; e < e1; e += BLOCK) plane[sg.ent_pos[e]] = ps[sg.ent_ig[e]];
+#pragma unroll
+        for (int f = 0; f < SPF; ++f)
+            if (spos[f] >= 0) plane[spos[f]] = sval[f];
+        {
+            const LocalSeg<T>& sg = *cur.sg;
+            const C* ps = sg.psi + (int64_t)cur.b * sg.ld;
+            const int e1 = sg.plane_ptr[cur.p + 1];
+            for (int e = sg.plane_ptr[cur.p] + threadIdx.x + SPF * BLOCK; e < e1; e += BLOCK)
+                plane[sg.ent_pos[e]] = ps[sg.ent_ig[e]];
+        }
         __syncthreads();
-        ct_fft_rows<N, +1, true, BLOCK, T>(plane, tw, srows, nrows);   // along k, rows with data
+        if (nxt < nitems) fetch_rows(q);
+        // along k, rows with data
+        ct_stage_rows<N, Plan2<N>::P1, 1, +1, true, BLOCK, T>(plane, tw, srows, nr);
+        __syncthreads();
+        if (nxt < nitems) fetch_index(q);
+        ct_stage_rows<N, Plan2<N>::P2, Plan2<N>::P1, +1, true, BLOCK, T>(plane, tw, srows, nr);
+        __syncthreads();
         // along j (columns): first stage in LDS, the last one stores the rows
         // (pad columns of FP32 rows of 128-byte pitch carry junk)
         ct_stage<N, Plan2<N>::P1, 1, +1, true, N, 1, N, BLOCK, T>(plane, tw);
         __syncthreads();
-        ct_cols_last_to_global<N, WP, +1, BLOCK, T>(plane, tw, sg.work + ((int64_t)b * sg.nplanes + p) * PS);
+        if (nxt < nitems) fetch_values(q);
+        ct_cols_last_to_global<N, WP, +1, BLOCK, T>(
+            plane, tw, cur.sg->work + ((int64_t)cur.b * cur.sg->nplanes + cur.p) * PS);
         __syncthreads();
     }
 }
 
 template <typename T, int N, int BLOCK>
-__global__ void __launch_bounds__(BLOCK)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PlaneTune<T>::R2GW)))
 k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
                const cx_t<T>* __restrict__ tw_g) {
     using C = cx_t<T>;
     constexpr int N2 = N * N;
     constexpr int WP = ws_pitch<T>(N);
     constexpr int64_t PS = ws_plane<T>(N, N);
-    constexpr int PF = 2;
+    constexpr int GPF = PlaneTune<T>::GPF;
     using Cs = ColStage<N, N, BLOCK>;
+    // column-stage inputs u < YP of the next plane load during the gather,
+    // the rest at the top of the item
+    constexpr int YP = PlaneTune<T>::YPF < Cs::P2 ? PlaneTune<T>::YPF : Cs::P2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* plane = reinterpret_cast<C*>(smem_raw);
     C* tw = plane + N2;
     int* srows = reinterpret_cast<int*>(tw + N);        // rows of the plane that are gathered
     if (threadIdx.x < N) tw[threadIdx.x] = tw_g[threadIdx.x];
     __syncthreads();                                    // tw is read before the first item's barrier
-    auto plane_in = [&](int it) {
-        const LocalSeg<T>& sg = bt.s[seg_of(bt, it)];
-        const int local = it - sg.item0;
-        const int p = local % sg.nplanes, b = local / sg.nplanes;
-        return static_cast<const C*>(sg.work) + ((int64_t)b * sg.nplanes + p) * PS;
-    };
-    // The first column stage (along j) runs on registers loaded from the
-    // workspace plane.  FP32 loads the next plane's behind this one's FFT;
-    // FP64 loads each plane's at the top of its item.
+    // the first column stage (along j) runs on registers loaded from the
+    // workspace plane, with the row list, one item ahead
     C y[Cs::ITER][Cs::P2];
+    int nrows = 0, srow = 0;
+    auto plane_of = [&](const PlaneItem<T>& q) {
+        return static_cast<const C*>(q.sg->work) + ((int64_t)q.b * q.sg->nplanes + q.p) * PS;
+    };
+    auto fetch = [&](const PlaneItem<T>& q) {
+        ct_cols_first_load<N, WP, BLOCK, T, 0, YP>(plane_of(q), y);
+        const int r0 = q.sg->row_ptr[q.p];
+        nrows = q.sg->row_ptr[q.p + 1] - r0;
+        if (threadIdx.x < nrows) srow = q.sg->rows[r0 + threadIdx.x];
+    };
     int item = blockIdx.x;
-    if constexpr (kPipeR2G<T>)
-        if (item < nitems) ct_cols_first_load<N, WP, BLOCK, T>(plane_in(item), y);
+    if (item < nitems) fetch(plane_item(bt, item));
     for (; item < nitems; item += gridDim.x) {
-        const LocalSeg<T>& sg = bt.s[seg_of(bt, item)];
-        const int local = item - sg.item0;
-        const int p = local % sg.nplanes, b = local / sg.nplanes;
-        if constexpr (!kPipeR2G<T>) ct_cols_first_load<N, WP, BLOCK, T>(plane_in(item), y);
+        const PlaneItem<T> cur = plane_item(bt, item);
+        const LocalSeg<T>& sg = *cur.sg;
+        if constexpr (YP < Cs::P2) ct_cols_first_load<N, WP, BLOCK, T, YP, Cs::P2>(plane_of(cur), y);
         ct_cols_first_to_lds<N, -1, BLOCK, T>(plane, tw, y);
-        const int r0 = sg.row_ptr[p], nrows = sg.row_ptr[p + 1] - r0;
-        if (threadIdx.x < nrows) srows[threadIdx.x] = sg.rows[r0 + threadIdx.x];
+        const int nr = nrows;
+        if (threadIdx.x < nr) srows[threadIdx.x] = srow;
         __syncthreads();
-        const int e0 = sg.plane_ptr[p], e1 = sg.plane_ptr[p + 1];
-        const C* ps = sg.psi + (int64_t)b * sg.ld;
-        C* hs = sg.hpsi + (int64_t)b * sg.ldo;
+        const int e0 = sg.plane_ptr[cur.p], e1 = sg.plane_ptr[cur.p + 1];
+        const C* ps = sg.psi + (int64_t)cur.b * sg.ld;
+        C* hs = sg.hpsi + (int64_t)cur.b * sg.ldo;
         const int* ent_ig = sg.ent_ig;
         const int* ent_pos = sg.ent_pos;
         const double* kin = sg.kin;
-        auto gather = [&](int e) {
+        // this item's first GPF gather entries: indices now, values one stage later
+        constexpr int GA = GPF > 0 ? GPF : 1;
+        int gi[GA], gp[GA];
+        C gs[GA], gh[GA];
+        double gk[GA];
+#pragma unroll
+        for (int f = 0; f < GPF; ++f) {
+            const int e = e0 + threadIdx.x + f * BLOCK;
+            gi[f] = -1;
+            if (e < e1) {
+                gi[f] = ent_ig[e];
+                gp[f] = ent_pos[e];
+            }
+        }
+        ct_stage<N, Cs::P1, 1, -1, false, N, 1, N, BLOCK, T>(plane, tw);   // along j, last stage
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < GPF; ++f) {
+            if (gi[f] >= 0) {
+                gs[f] = ps[gi[f]];
+                gk[f] = kin[gi[f]];
+                gh[f] = accumulate ? hs[gi[f]] : C{0, 0};
+            }
+        }
+        ct_fft_rows<N, -1, false, BLOCK, T>(plane, tw, srows, nr);        // along k, gathered rows
+        const int nxt = item + gridDim.x;
+        if (nxt < nitems) fetch(plane_item(bt, nxt));
+#pragma unroll
+        for (int f = 0; f < GPF; ++f) {
+            if (gi[f] < 0) continue;
+            const C v = plane[gp[f]];
+            const double re = gk[f] * (double)gs[f].x + (double)(v.x * scale) + (double)gh[f].x;
+            const double im = gk[f] * (double)gs[f].y + (double)(v.y * scale) + (double)gh[f].y;
+            hs[gi[f]] = C{(T)re, (T)im};
+        }
+#pragma unroll 1
+        for (int e = e0 + threadIdx.x + GPF * BLOCK; e < e1; e += BLOCK) {
             const int ig = ent_ig[e];
             const C v = plane[ent_pos[e]];
             const C sv = ps[ig];
@@ -300,44 +426,6 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
                 im += (double)h.y;
             }
             hs[ig] = C{(T)re, (T)im};
-        };
-        if constexpr (kPipeR2G<T>) {
-            // this item's gather inputs, then the next plane, both in flight
-            // during the FFT
-            int gi[PF], gp[PF];
-            C gs[PF], gh[PF];
-            double gk[PF];
-#pragma unroll
-            for (int f = 0; f < PF; ++f) {
-                const int e = e0 + threadIdx.x + f * BLOCK;
-                gi[f] = -1;
-                if (e < e1) {
-                    gi[f] = ent_ig[e];
-                    gp[f] = ent_pos[e];
-                    gs[f] = ps[gi[f]];
-                    gk[f] = kin[gi[f]];
-                    gh[f] = accumulate ? hs[gi[f]] : C{0, 0};
-                }
-            }
-            const int nxt = item + gridDim.x;
-            if (nxt < nitems) ct_cols_first_load<N, WP, BLOCK, T>(plane_in(nxt), y);
-            ct_stage<N, Cs::P1, 1, -1, false, N, 1, N, BLOCK, T>(plane, tw);   // along j, last stage
-            __syncthreads();
-            ct_fft_rows<N, -1, false, BLOCK, T>(plane, tw, srows, nrows); // along k, gathered rows
-#pragma unroll
-            for (int f = 0; f < PF; ++f) {
-                if (gi[f] < 0) continue;
-                const C v = plane[gp[f]];
-                double re = gk[f] * (double)gs[f].x + (double)(v.x * scale) + (double)gh[f].x;
-                double im = gk[f] * (double)gs[f].y + (double)(v.y * scale) + (double)gh[f].y;
-                hs[gi[f]] = C{(T)re, (T)im};
-            }
-            for (int e = e0 + threadIdx.x + PF * BLOCK; e < e1; e += BLOCK) gather(e);
-        } else {
-            ct_stage<N, Cs::P1, 1, -1, false, N, 1, N, BLOCK, T>(plane, tw);   // along j, last stage
-            __syncthreads();
-            ct_fft_rows<N, -1, false, BLOCK, T>(plane, tw, srows, nrows); // along k, gathered rows
-            for (int e = e0 + threadIdx.x; e < e1; e += BLOCK) gather(e);
         }
         __syncthreads();
     }
