@@ -432,7 +432,7 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
 }
 
 // x pencils of one (j, z-chunk) for NBW consecutive bands of one segment,
-// as register FFTs.  N1 = P1 * P2 (Plan2).  Thread (z, role):
+// as register FFTs.  N1 = P1 * P2 (Plan2, P1 >= P2).  Thread (z, role):
 //   G2R (DIT)  role b < P2 loads x = P2 u + b (u < P1) straight from the
 //              plane workspace, DFT_P1 -> LDS transpose -> role k1 < P1
 //              gathers Z[u'][k1], twiddle W^(u' k1), DFT_P2: it now holds
@@ -441,24 +441,41 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
 //   R2G (DIF)  DFT_P2 on the same values, twiddle W^(q k1) -> LDS transpose
 //              -> role b gathers, DFT_P1 -> holds x = P2 q + b, stored to
 //              the active planes only.
-// Two LDS round trips per band (double-buffered transposes, two barriers);
-// the next band's loads are issued before the current band's arithmetic.
+// Two LDS round trips per band through one N1 x TZ buffer, two barriers:
+// each thread's LDS writes of a phase go to exactly the slots it read in the
+// phase before ((u P1 + k1) for role k1 in the middle phase, (b P1 + q) for
+// role b in the outer ones), so no write can overtake another thread's read
+// and one buffer serves all three phases.
+// The kernel is VALU-bound in FP32, so the per-band instruction count is
+// kept close to the FFT itself: the x -> (plane) byte offsets and active
+// flags of a thread's P1 pencil points are band-invariant and computed once
+// (HOIST; a band's loads and stores are then a uniform base plus a 32-bit
+// offset), and twiddle W^0 = 1 is multiplied rather than selected.  FP64 is
+// bound by HBM and occupancy instead and recomputes the offsets per band
+// from the plane map in LDS, which keeps it at 148 VGPRs (three waves per
+// SIMD; hoisted, 182).  Measured on 90^3 (256 bands): FP32 429 -> 358 us
+// with HOIST, 445 us without; FP64 665 us either way at 148 VGPRs, 746 us
+// hoisted, 722 us with V_loc in LDS instead of registers (VR).  Running two
+// bands per loop trip on two register sets, without the copy below, needs
+// about 60 more VGPRs and was slower in both precisions.
 // Grid: (n2 * z-chunks, nseg * band groups).
+template <typename T, int N1>
+struct PencilShape;
+
 template <typename T, int N1, int TZ, int NBW>
-__global__ void __launch_bounds__(TZ * cmax(Plan2<N1>::P1, Plan2<N1>::P2))
+__global__ void __launch_bounds__(TZ * Plan2<N1>::P1)
+__attribute__((amdgpu_waves_per_eu(PencilShape<T, N1>::W)))
 k_xpencil_vloc_ct(const LocalBatch<T> bt, const T* __restrict__ vloc, int n2, int n3,
                   const cx_t<T>* __restrict__ tw_g) {
     using C = cx_t<T>;
     constexpr int P1 = Plan2<N1>::P1, P2 = Plan2<N1>::P2;
-    __shared__ C ex[2][N1 * TZ];
+    static_assert(P1 >= P2, "roles = P1");
+    __shared__ C ex[N1 * TZ];
     __shared__ C tw[N1];
-    __shared__ int srcpl[N1];
     const int nbg = (bt.nb + NBW - 1) / NBW;
     const int sidx = blockIdx.y / nbg;
     const int bgrp = blockIdx.y - sidx * nbg;
     const LocalSeg<T>& sg = bt.s[sidx];
-    const int nplanes = sg.nplanes;
-    C* work = sg.work;
     const int nzc = (n3 + TZ - 1) / TZ;
     const int j = blockIdx.x / nzc;
     const int z0 = (blockIdx.x - j * nzc) * TZ;
@@ -466,85 +483,128 @@ k_xpencil_vloc_ct(const LocalBatch<T> bt, const T* __restrict__ vloc, int n2, in
     const int role = threadIdx.x / TZ;
     const bool zv = z0 + z < n3;
     const int64_t n23 = (int64_t)n2 * n3;
-    for (int i = threadIdx.x; i < N1; i += TZ * cmax(P1, P2)) {
-        tw[i] = tw_g[i];
-        srcpl[i] = sg.x_to_plane[i];
-    }
-    // V at the natural x = role + P1 q' this thread scales
+    for (int i = threadIdx.x; i < N1; i += TZ * P1) tw[i] = tw_g[i];
+    // V at the natural x = role + P1 q this thread scales: in registers
+    // (VR) or in LDS (x * TZ + z), which frees 2 P2 FP64 registers
+    constexpr bool VR = PencilShape<T, N1>::VR;
+    __shared__ T vs[VR ? 1 : N1 * TZ];
     T vr[P2];
+    if constexpr (VR) {
 #pragma unroll
-    for (int q = 0; q < P2; ++q) {
-        const int x = role + P1 * q;
-        vr[q] = (role < P1 && zv) ? vloc[(int64_t)x * n23 + (int64_t)j * n3 + z0 + z] : T(0);
+        for (int q = 0; q < P2; ++q) {
+            const int x = role + P1 * q;
+            vr[q] = zv ? vloc[(int64_t)x * n23 + (int64_t)j * n3 + z0 + z] : T(0);
+        }
+    } else {
+        for (int i = threadIdx.x; i < N1 * TZ; i += TZ * P1) {
+            const int x = i / TZ, zz = i - x * TZ;
+            vs[i] = z0 + zz < n3 ? vloc[(int64_t)x * n23 + (int64_t)j * n3 + z0 + zz] : T(0);
+        }
     }
-    __syncthreads();
-    const int64_t wps = ws_plane<T>(n2, n3);           // workspace plane stride
-    const int64_t col = (int64_t)j * ws_pitch<T>(n3) + z0 + z;
-    C y[P1];
-    auto load = [&](int b) {
-        const C* wb = work + (int64_t)b * nplanes * wps + col;
+    // this thread's pencil points x = P2 u + role: byte offset inside a
+    // band's workspace and whether the plane is stored (active) at all
+    const int64_t wps = ws_plane<T>(n2, n3);
+    const int64_t band_bytes = (int64_t)sg.nplanes * wps * (int64_t)sizeof(C);
+    const unsigned colb = (unsigned)(((int64_t)j * ws_pitch<T>(n3) + z0 + z) * (int64_t)sizeof(C));
+    // (inactive: offset kNone).  HOIST keeps them in registers across the
+    // bands; otherwise each band recomputes them from the plane map in LDS.
+    constexpr bool HOIST = PencilShape<T, N1>::HOIST;
+    constexpr unsigned kNone = ~0u;
+    __shared__ int srcpl[HOIST ? 1 : N1];
+    const unsigned plane_bytes = (unsigned)(wps * (int64_t)sizeof(C));
+    unsigned boff[P1];
+    bool actr[P1];
+    if constexpr (HOIST) {
 #pragma unroll
         for (int u = 0; u < P1; ++u) {
-            const int x = P2 * u + role;
-            const int pl = (role < P2) ? srcpl[x] : -1;
-            y[u] = (pl >= 0 && zv) ? wb[(int64_t)pl * wps] : C{0, 0};
+            const int pl = (role < P2 && zv) ? sg.x_to_plane[P2 * u + role] : -1;
+            actr[u] = pl >= 0;
+            boff[u] = pl >= 0 ? (unsigned)pl * plane_bytes + colb : kNone;
+        }
+    } else {
+        for (int i = threadIdx.x; i < N1; i += TZ * P1) srcpl[i] = sg.x_to_plane[i];
+    }
+    auto off = [&](int u) {
+        if constexpr (HOIST) return boff[u];
+        const int pl = (role < P2 && zv) ? srcpl[P2 * u + role] : -1;
+        return pl >= 0 ? (unsigned)pl * plane_bytes + colb : kNone;
+    };
+    auto act = [&](int u, unsigned o) { return HOIST ? actr[u] : o != kNone; };
+    __syncthreads();
+    const char* work = reinterpret_cast<const char*>(sg.work);
+    auto load = [&](int b, C (&y)[P1]) {
+        const char* wb = work + (int64_t)b * band_bytes;
+#pragma unroll
+        for (int u = 0; u < P1; ++u) {
+            const unsigned o = off(u);
+            y[u] = act(u, o) ? *reinterpret_cast<const C*>(wb + o) : C{0, 0};
         }
     };
-    const int bfirst = bgrp * NBW;
-    const int bend = min(bt.nb, bfirst + NBW);
-    if (bfirst < bend) load(bfirst);
-    for (int b = bfirst; b < bend; ++b) {
-        C v[P1];
-#pragma unroll
-        for (int u = 0; u < P1; ++u) v[u] = y[u];
-        if (b + 1 < bend) load(b + 1);                      // prefetch the next band
+    auto band = [&](int b, C (&y)[P1]) {
         // ---- G2R stage 1: DFT_P1 over u, positions b*P1 + q
         if (role < P2) {
-            dft<P1, +1, T>(v);
+            dft<P1, +1, T>(y);
 #pragma unroll
-            for (int q = 0; q < P1; ++q) ex[0][(role * P1 + q) * TZ + z] = v[q];
+            for (int q = 0; q < P1; ++q) ex[(role * P1 + q) * TZ + z] = y[q];
         }
         __syncthreads();
-        C w[P2];
-        if (role < P1) {
+        {
             // ---- G2R stage 2: twiddle W^(u' k1) (backward), DFT_P2
+            C w[P2];
 #pragma unroll
             for (int u = 0; u < P2; ++u) {
-                C a = ex[0][(u * P1 + role) * TZ + z];
-                w[u] = (u * role) ? cmul(a, twid<+1>(tw, u * role)) : a;
+                const C a = ex[(u * P1 + role) * TZ + z];
+                w[u] = u == 0 ? a : cmul(a, twid<+1>(tw, u * role));
             }
             dft<P2, +1, T>(w);
             // ---- V_loc(r) at natural x = role + P1 q
 #pragma unroll
-            for (int q = 0; q < P2; ++q) w[q] = C{w[q].x * vr[q], w[q].y * vr[q]};
+            for (int q = 0; q < P2; ++q) {
+                const T vq = VR ? vr[q] : vs[(role + P1 * q) * TZ + z];
+                w[q] = C{w[q].x * vq, w[q].y * vq};
+            }
             // ---- R2G stage 1 (DIF): DFT_P2, twiddle W^(q k1) (forward)
             dft<P2, -1, T>(w);
 #pragma unroll
-            for (int q = 0; q < P2; ++q) {
-                C a = (q * role) ? cmul(w[q], twid<-1>(tw, q * role)) : w[q];
-                ex[1][(q * P1 + role) * TZ + z] = a;
-            }
+            for (int q = 0; q < P2; ++q)
+                ex[(q * P1 + role) * TZ + z] = q == 0 ? w[q] : cmul(w[q], twid<-1>(tw, q * role));
         }
         __syncthreads();
-        if (role < P2 && zv) {
+        if (role < P2) {
             // ---- R2G stage 2: DFT_P1, natural x = P2 q + b
 #pragma unroll
-            for (int u = 0; u < P1; ++u) v[u] = ex[1][(role * P1 + u) * TZ + z];
-            dft<P1, -1, T>(v);
-            C* wb = work + (int64_t)b * nplanes * wps + col;
+            for (int u = 0; u < P1; ++u) y[u] = ex[(role * P1 + u) * TZ + z];
+            dft<P1, -1, T>(y);
+            char* wb = const_cast<char*>(work) + (int64_t)b * band_bytes;
 #pragma unroll
             for (int q = 0; q < P1; ++q) {
-                const int pl = srcpl[P2 * q + role];
-                if (pl >= 0) wb[(int64_t)pl * wps] = v[q];
+                const unsigned o = off(q);
+                if (act(q, o)) *reinterpret_cast<C*>(wb + o) = y[q];
             }
         }
+    };
+    const int bfirst = bgrp * NBW;
+    const int bend = min(bt.nb, bfirst + NBW);
+    C y[P1], v[P1];
+    if (bfirst < bend) load(bfirst, y);
+    for (int b = bfirst; b < bend; ++b) {
+#pragma unroll
+        for (int u = 0; u < P1; ++u) v[u] = y[u];
+        if (b + 1 < bend) load(b + 1, y);                  // the next band's loads fly meanwhile
+        band(b, v);
     }
 }
 
 // z-chunk width of the pencil kernel (measured: FP32 prefers chunks of one
 // 128-byte line, FP64 on the 90-point grid five chunks of 18)
-template <typename T, int N> struct PencilShape { static constexpr int TZ = 16; };
-template <> struct PencilShape<double, 90> { static constexpr int TZ = 18; };
+template <typename T, int N> struct PencilShape {
+    static constexpr int TZ = 16, W = 1;
+    static constexpr bool VR = true, HOIST = sizeof(T) == 4;
+};
+template <> struct PencilShape<double, 90> {
+    static constexpr int TZ = 18, W = 1;
+    static constexpr bool VR = true, HOIST = false;
+};
 
 
 static int Plan2N(int n, int which) {
@@ -705,7 +765,10 @@ hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, 
     // compile-time plans for square planes / pencil lengths of EFFM_CT_SIZES
     const bool plct = c.n2 == c.n3 && Plan2N(c.n2, 1) > 0 && plan_is(c.ax2, Plan2N(c.n2, 1), Plan2N(c.n2, 2)) &&
                       plan_is(c.ax3, Plan2N(c.n3, 1), Plan2N(c.n3, 2));
-    const bool pect = Plan2N(c.n1, 1) > 0 && plan_is(c.ax1, Plan2N(c.n1, 1), Plan2N(c.n1, 2));
+    // (the specialised pencil kernel addresses one band's workspace with
+    // 32-bit byte offsets)
+    const bool pect = Plan2N(c.n1, 1) > 0 && plan_is(c.ax1, Plan2N(c.n1, 1), Plan2N(c.n1, 2)) &&
+                      (int64_t)c.n1 * ws_plane<T>(c.n2, c.n3) * (int64_t)sizeof(C) < (int64_t{1} << 32);
     if (!plct || !pect) {
         // generic kernels: dynamic LDS sized by this cell's grid
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_g2r<T>),
