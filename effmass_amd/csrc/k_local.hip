@@ -208,22 +208,25 @@ k_plane_r2g(const LocalBatch<T> bt, T scale, int accumulate, FftAxis a2, FftAxis
 //        during the first column stage, psi / kin / hpsi after it);
 //   YPF  R2G: column-stage inputs u < YPF of the next plane, during the
 //        gather (the rest at the top of the item);
-//   G2RW / R2GW  waves per SIMD the register allocation must allow (8 lets
-//        two FP32 workgroups of 15 waves share a CU).
+//   G2RW / R2GW  waves per SIMD the register allocation must allow;
+//   BLOCK_CAP  largest workgroup: FP32 planes (65 KB of LDS) run two
+//        workgroups of 8 waves per CU, each thread taking up to two
+//        butterflies per stage, where 15-wave workgroups would need <= 64
+//        VGPRs for a second one.
 // Entries past SPF / GPF per thread (the densest planes) load directly.
 // Measured on the 90^3 grid (256 bands): FP64 R2G 420 -> 388 us with GPF 1;
-// FP32 R2G 239 -> 228 us at two workgroups per CU, where prefetching costs
-// the registers that allow the second; YPF > 0 spills in FP64 and did not
-// pay in FP32; SPF changes G2R by 1-3%.
+// FP32 G2R 242 -> 220 us with 512-thread workgroups, FP32 R2G 236 -> 228 us
+// at two workgroups per CU (either shape); YPF > 0 spills in FP64 and did
+// not pay in FP32; SPF changes G2R by 1-3%.
 template <typename T>
 struct PlaneTune;
 template <>
 struct PlaneTune<double> {
-    static constexpr int G2RW = 1, R2GW = 1, SPF = 1, GPF = 1, YPF = 0;
+    static constexpr int G2RW = 1, R2GW = 1, SPF = 1, GPF = 1, YPF = 0, BLOCK_CAP = 1024;
 };
 template <>
 struct PlaneTune<float> {
-    static constexpr int G2RW = 1, R2GW = 8, SPF = 2, GPF = 0, YPF = 0;
+    static constexpr int G2RW = 1, R2GW = 1, SPF = 2, GPF = 2, YPF = 0, BLOCK_CAP = 512;
 };
 
 // Flat item -> (segment, plane, band).
@@ -642,7 +645,7 @@ static hipError_t persistent_grid(Ctx& c, const void* fn, size_t lds, int block,
 template <typename T, int N>
 static hipError_t launch_planes_ct(Ctx& c, const LocalBatch<T>& bt, int nitems, bool accumulate,
                                    bool g2r, const cx_t<T>* tw) {
-    constexpr int BLOCK = plane_block<N>();
+    constexpr int BLOCK = cmin(plane_block<N>(), PlaneTune<T>::BLOCK_CAP);
     const size_t lds = sizeof(cx_t<T>) * ((size_t)N * N + N) + sizeof(int) * N;
     hipError_t e;
     int grid = 0;
