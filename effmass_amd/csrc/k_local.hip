@@ -453,14 +453,16 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
 // kept close to the FFT itself: the x -> (plane) byte offsets and active
 // flags of a thread's P1 pencil points are band-invariant and computed once
 // (HOIST; a band's loads and stores are then a uniform base plus a 32-bit
-// offset), and twiddle W^0 = 1 is multiplied rather than selected.  FP64 is
-// bound by HBM and occupancy instead and recomputes the offsets per band
-// from the plane map in LDS, which keeps it at 148 VGPRs (three waves per
-// SIMD; hoisted, 182).  Measured on 90^3 (256 bands): FP32 429 -> 358 us
-// with HOIST, 445 us without; FP64 665 us either way at 148 VGPRs, 746 us
-// hoisted, 722 us with V_loc in LDS instead of registers (VR).  Running two
-// bands per loop trip on two register sets, without the copy below, needs
-// about 60 more VGPRs and was slower in both precisions.
+// offset), and twiddle W^0 = 1 is multiplied rather than selected.  The
+// loads of a band are not issued ahead of the previous band's arithmetic
+// (BPF): without that register set the kernel fits 126 VGPRs in FP64 and
+// 81 in FP32, i.e. 5 resident workgroups per CU instead of 4 (FP64) and 8
+// instead of 6 (FP32), which hide the latency better.  Measured on 90^3
+// (256 bands): FP32 429 us -> 358 us (HOIST) -> 348 us (no BPF); FP64
+// 665 us -> 594 us (HOIST, no BPF; with BPF, hoisting costs a wave per
+// SIMD and 746 us); V_loc in LDS instead of registers (VR) was slower.
+// Running two bands per loop trip on two register sets needs about 60 more
+// VGPRs and was slower in both precisions.
 // Grid: (n2 * z-chunks, nseg * band groups).
 template <typename T, int N1>
 struct PencilShape;
@@ -588,13 +590,23 @@ k_xpencil_vloc_ct(const LocalBatch<T> bt, const T* __restrict__ vloc, int n2, in
     };
     const int bfirst = bgrp * NBW;
     const int bend = min(bt.nb, bfirst + NBW);
-    C y[P1], v[P1];
-    if (bfirst < bend) load(bfirst, y);
-    for (int b = bfirst; b < bend; ++b) {
+    // BPF: the next band's loads fly during this band's arithmetic (P1 more
+    // complex registers); otherwise more resident workgroups hide them
+    C v[P1];
+    if constexpr (PencilShape<T, N1>::BPF) {
+        C y[P1];
+        if (bfirst < bend) load(bfirst, y);
+        for (int b = bfirst; b < bend; ++b) {
 #pragma unroll
-        for (int u = 0; u < P1; ++u) v[u] = y[u];
-        if (b + 1 < bend) load(b + 1, y);                  // the next band's loads fly meanwhile
-        band(b, v);
+            for (int u = 0; u < P1; ++u) v[u] = y[u];
+            if (b + 1 < bend) load(b + 1, y);
+            band(b, v);
+        }
+    } else {
+        for (int b = bfirst; b < bend; ++b) {
+            load(b, v);
+            band(b, v);
+        }
     }
 }
 
@@ -602,11 +614,11 @@ k_xpencil_vloc_ct(const LocalBatch<T> bt, const T* __restrict__ vloc, int n2, in
 // 128-byte line, FP64 on the 90-point grid five chunks of 18)
 template <typename T, int N> struct PencilShape {
     static constexpr int TZ = 16, W = 1;
-    static constexpr bool VR = true, HOIST = sizeof(T) == 4;
+    static constexpr bool VR = true, HOIST = true, BPF = false;
 };
 template <> struct PencilShape<double, 90> {
     static constexpr int TZ = 18, W = 1;
-    static constexpr bool VR = true, HOIST = false;
+    static constexpr bool VR = true, HOIST = true, BPF = false;
 };
 
 

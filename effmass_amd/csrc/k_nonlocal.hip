@@ -501,7 +501,9 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     // Slabs of BK k-rows of A (64 consecutive m) and of B (64 consecutive n)
     // are staged through LDS, double-buffered; each global load instruction
     // reads one 64-element row segment (whole cache lines).
-    constexpr int BK = 128 / (int)sizeof(C);          // FP32 16, FP64 8
+    // (FP64 256-byte slabs: measured 680 -> 637 us on the 64-atom apply; in
+    // FP32 the doubled LDS would halve the workgroups per CU)
+    constexpr int BK = (sizeof(T) == 8 ? 256 : 128) / (int)sizeof(C);   // FP32 16, FP64 16
     // row stride of 64 elements + 128 bytes: consecutive k-rows start 32
     // banks apart, so the two 16-lane halves of an 8-byte read cycle do not
     // collide
