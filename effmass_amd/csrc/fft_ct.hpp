@@ -135,6 +135,19 @@ struct ColStage {
     static constexpr int ITER = (TOTAL + BLOCK - 1) / BLOCK;
 };
 
+// Streaming (nontemporal) store of one complex value: the plane workspace
+// is written once and read by the next kernel, so it need not stay in the
+// caches.  (Measured on the 90^3 workspace, 1.36 GB FP64: the G2R pass 1-3%
+// faster and the x-pencil pass that reads it 2-3% faster than with plain
+// stores; nontemporal stores in the x-pencil pass itself were slower.)
+template <typename C>
+__device__ __forceinline__ void store_stream(C v, C* p) {
+    using E = decltype(v.x);
+    typedef E v2 __attribute__((ext_vector_type(2)));
+    const v2 w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
+}
+
 // G2R: last DIT stage of the columns, from LDS to out[row * WP + col].
 // The pad columns N <= col < WP repeat column N - 1 (whole rows are written).
 template <int N, int WP, int S, int BLOCK, typename T>
@@ -155,7 +168,7 @@ __device__ __forceinline__ void ct_cols_last_to_global(const cx_t<T>* buf, const
             }
             dft<Cs::P2, S, T>(y);
 #pragma unroll
-            for (int u = 0; u < Cs::P2; ++u) out[(int64_t)(r + u * Cs::P1) * WP + col] = y[u];
+            for (int u = 0; u < Cs::P2; ++u) store_stream(y[u], &out[(int64_t)(r + u * Cs::P1) * WP + col]);
         }
     }
 }
