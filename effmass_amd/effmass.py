@@ -71,8 +71,10 @@ def velocity_matrices(H: Hamiltonian, ikpt: int, X: torch.Tensor, prec="fp64"):
     Xp = X.to(torch.complex128 if prec == "fp64" else torch.complex64).contiguous()
     D = [torch.zeros_like(Xp) for _ in range(3)]
     H.gradient().at_k(ikpt)(Xp, GradHPsi(*D), prec=prec, overwrite=True)
-    X64 = X.to(torch.complex128)
-    return [(X64.conj() @ d.to(torch.complex128).T).cpu().numpy() for d in D]
+    X64 = X.to(torch.complex128).contiguous()
+    # FP64 split-K products on the engine (a long-K complex128 matmul on
+    # the BLAS library runs at a fraction of the MFMA rate)
+    return [H.overlap(X64, d.to(torch.complex128)).cpu().numpy() for d in D]
 
 
 def d2h_expect(H: Hamiltonian, kp: synth.KPoint, A, X: torch.Tensor, bands, prec="fp64",
