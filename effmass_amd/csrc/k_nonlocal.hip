@@ -501,13 +501,16 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     // Slabs of BK k-rows of A (64 consecutive m) and of B (64 consecutive n)
     // are staged through LDS, double-buffered; each global load instruction
     // reads one 64-element row segment (whole cache lines).
-    // (FP64 256-byte slabs: measured 680 -> 637 us on the 64-atom apply; in
-    // FP32 the doubled LDS would halve the workgroups per CU)
-    constexpr int BK = (sizeof(T) == 8 ? 256 : 128) / (int)sizeof(C);   // FP32 16, FP64 16
-    // row stride of 64 elements + 128 bytes: consecutive k-rows start 32
-    // banks apart, so the two 16-lane halves of an 8-byte read cycle do not
-    // collide
-    constexpr int LDM = 64 + 128 / (int)sizeof(C);
+    // FP64: 12 k-rows per slab and unpadded rows, 48 KB, so three
+    // workgroups share a CU (measured on the 64-atom apply: 8 rows 680 us,
+    // 16 rows 640 us at two workgroups per CU, 12 rows 590 us); FP32: 16
+    // rows (wider slabs halve the workgroups per CU, narrower add barriers)
+    constexpr int BK = sizeof(T) == 8 ? 12 : 16;
+    // row stride: a 16-byte read cycle (8 lanes) covers 128 contiguous bytes
+    // of one k-row, so FP64 needs no padding; FP32 rows are 64 elements +
+    // 128 bytes, consecutive k-rows start 32 banks apart and the two 16-lane
+    // halves of an 8-byte read cycle do not collide
+    constexpr int LDM = sizeof(T) == 8 ? 64 : 64 + 128 / (int)sizeof(C);
     constexpr int EPT = 64 * BK / 256;                // elements per thread, matrix and slab
     __shared__ __align__(16) C sA[2][BK * LDM];
     __shared__ __align__(16) C sB[2][BK * LDM];
