@@ -176,11 +176,14 @@ def main():
                     ent.update(achieved=ach, peak=pk, unit="TFLOP/s", frac=ach / pk, algorithmic_flops=amount)
             kernels.append(ent)
         H.set_timing(False)
-    dom = max((k for k in kernels if k["prec"] == "mixed" and "frac" in k), key=lambda k: k["share_of_step"])
-    traffic = measured_traffic(dom["kernel"], "mixed")
-    roofline = {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
-                "frac": dom["frac"], "traffic": traffic, "kernel": dom["kernel"], "prec": "mixed",
-                "traffic_source": TRAFFIC_PROFILE if traffic is not None else None}
+    dom = max((k for k in kernels if k["prec"] == "mixed" and "frac" in k), key=lambda k: k["share_of_step"],
+              default=None)
+    roofline = None                       # needs the per-kernel timing steps (--timing-steps > 0)
+    if dom is not None:
+        traffic = measured_traffic(dom["kernel"], "mixed")
+        roofline = {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
+                    "frac": dom["frac"], "traffic": traffic, "kernel": dom["kernel"], "prec": "mixed",
+                    "traffic_source": TRAFFIC_PROFILE if traffic is not None else None}
 
     # CPU baseline: the FP64 oracle (reference loop structure) on host cores
     cpu = None
@@ -270,10 +273,11 @@ def main():
 
 def config2_leg(steps, warmup=1):
     """H|psi> and dH/dk_{x,y,z}|psi> for 32 k-points (4x4x2 grid, ragged npw
-    36,816-37,073) x 128 bands of the 64-atom cell: effm_hpsi_multi and
-    effm_dhdk_multi (up to 16 k-points per launch), replayed as captured
-    hipGraphs; psi is seeded device noise (throughput does not depend on
-    the values).  Also times the same work as one call per k-point."""
+    36,816-37,073) x 128 bands of the 64-atom cell: effm_hpsi_dhdk_multi
+    (one projection GEMM for both, up to 16 k-points per launch), replayed
+    as a captured hipGraph; psi is seeded device noise (throughput does not
+    depend on the values).  Also times the same work as effm_hpsi_multi +
+    effm_dhdk_multi (graph and eager) and as one call per k-point."""
     import torch
     from effmass_amd import synth
     from effmass_amd.hamiltonian import GradHPsi, Hamiltonian
@@ -294,6 +298,9 @@ def config2_leg(steps, warmup=1):
                       [torch.empty_like(p) for p in psis])
 
         def step(graph=True):
+            H.apply_with_gradient_multi(ks, psis, hs, gr, prec=prec, overwrite=True, graph=graph)
+
+        def separate(graph=True):
             H.apply_multi(ks, psis, hs, prec=prec, overwrite=True, graph=graph)
             H.gradient_multi(ks, psis, gr, prec=prec, overwrite=True, graph=graph)
 
@@ -315,6 +322,7 @@ def config2_leg(steps, warmup=1):
 
         t_graph = timed(step, steps)
         t_eager = timed(lambda: step(False), steps)
+        t_sep = timed(separate, steps)
         t_loop = timed(per_k, steps)
         # per-kernel split of one eager batched step (HIP events)
         H.reset_timing()
@@ -329,8 +337,8 @@ def config2_leg(steps, warmup=1):
         H.set_timing(False)
         applies = len(ks) * nb
         res[prec] = {"ms_per_step": t_graph * 1e3, "k_band_steps_per_s": applies / t_graph,
-                     "eager_batched_ms": t_eager * 1e3, "per_k_calls_ms": t_loop * 1e3,
-                     "kernel_ms": split}
+                     "eager_batched_ms": t_eager * 1e3, "separate_calls_graph_ms": t_sep * 1e3,
+                     "per_k_calls_ms": t_loop * 1e3, "kernel_ms": split}
         del psis, hs, gr
         torch.cuda.empty_cache()
     H.close()

@@ -289,14 +289,31 @@ static int run_hpsi(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool acc, bo
 }
 
 template <typename T>
-static int run_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool acc, bool launch) {
+// with_h: H|psi> into each segment's hout as well, its nonlocal part from
+// the <p|psi> block of the gradient's projection (one projection GEMM for
+// both instead of two).
+static int run_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool acc, bool launch, bool with_h = false) {
     using C = cx_t<T>;
     int rc;
     for (int i = 0; i < nseg; ++i)
         if ((rc = ensure_proj<T>(c, *const_cast<KptDev*>(segs[i].k), true))) return rc;
+    const size_t cap = work_cap(c);
     for (int g0 = 0; g0 < nseg; g0 += kMaxSeg) {
         const int ng = std::min(kMaxSeg, nseg - g0);
         const ApplySeg* sg = segs + g0;
+        // the local part of H|psi>: segments writing to hout
+        ApplySeg hs[kMaxSeg];
+        int chunk = 0;
+        if (with_h) {
+            for (int i = 0; i < ng; ++i) {
+                hs[i] = sg[i];
+                hs[i].out[0] = sg[i].hout;
+                hs[i].out[1] = hs[i].out[2] = nullptr;
+            }
+            const size_t wpb = std::max<size_t>(1, local_work_bytes<T>(c, hs, ng, 1));
+            chunk = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nb, cap / wpb, (size_t)65535}));
+            if ((rc = ensure(c, c.work, wpb * chunk, "FFT workspace"))) return rc;
+        }
         if (c.nch() > 0) {
             int mn = sg[0].k->npw;
             for (int i = 1; i < ng; ++i) mn = std::min(mn, sg[i].k->npw);
@@ -306,7 +323,13 @@ static int run_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool acc, bo
             if ((rc = ensure(c, c.cbuf, (size_t)ng * 4 * c.nch() * nb * sizeof(C), "projections"))) return rc;
         }
         if (!launch) continue;
-        HIP_TRY(launch_dhdk<T>(c, sg, ng, nb, acc, c.partial.ptr, c.partial.bytes, c.cbuf.ptr), "dH/dk kernels");
+        if (with_h)
+            for (int b0 = 0; b0 < nb; b0 += chunk) {
+                const int nbc = std::min(chunk, nb - b0);
+                HIP_TRY(launch_local<T>(c, hs, ng, b0, nbc, acc, c.work.ptr), "local potential kernels");
+            }
+        HIP_TRY(launch_dhdk<T>(c, sg, ng, nb, acc, c.partial.ptr, c.partial.bytes, c.cbuf.ptr, with_h),
+                "dH/dk kernels");
     }
     return kOk;
 }
@@ -461,6 +484,18 @@ static int apply_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool acc, 
     if (rc) return rc;
     return run_maybe_graph(c, graph, graph ? graph_key(1, prec, acc, nb, segs, nseg) : std::vector<int64_t>{},
                            [&] { return run_dhdk<T>(c, segs, nseg, nb, acc, true); });
+}
+
+template <typename T>
+static int apply_hd(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool acc, bool graph, int prec) {
+    int rc = run_dhdk<T>(c, segs, nseg, nb, acc, false, true);
+    if (rc) return rc;
+    std::vector<int64_t> key;
+    if (graph) {
+        key = graph_key(2, prec, acc, nb, segs, nseg);
+        for (int i = 0; i < nseg; ++i) key.push_back((int64_t)(intptr_t)segs[i].hout);
+    }
+    return run_maybe_graph(c, graph, std::move(key), [&] { return run_dhdk<T>(c, segs, nseg, nb, acc, true, true); });
 }
 
 }  // namespace effm
@@ -919,6 +954,27 @@ int effm_dhdk_multi(effm_ctx* h, int nk, const int* ikpts, int nb, const void* c
     const bool graph = (flags & EFFM_F_GRAPH) != 0;
     return prec == EFFM_PREC_FP64 ? apply_dhdk<double>(c, segs.data(), nk, nb, acc, graph, prec)
                                   : apply_dhdk<float>(c, segs.data(), nk, nb, acc, graph, prec);
+}
+
+int effm_hpsi_dhdk_multi(effm_ctx* h, int nk, const int* ikpts, int nb, const void* const* psi_ptrs,
+                         const int64_t* ld, void* const* hpsi_ptrs, void* const* ox, void* const* oy,
+                         void* const* oz, int prec, int flags) {
+    if (!h) return kErrInvalid;
+    Ctx& c = h->c;
+    (void)hipSetDevice(c.device);
+    std::vector<ApplySeg> segs;
+    int rc = multi_segments(c, nk, ikpts, nb, psi_ptrs, ld, ox, oy, oz, prec, flags, segs);
+    if (rc) return rc;
+    if (nk > 0 && !hpsi_ptrs) return fail(c, kErrInvalid, "hpsi pointers must not be NULL");
+    for (int i = 0; i < nk; ++i) {
+        if (!hpsi_ptrs[i]) return fail(c, kErrInvalid, "hpsi must not be NULL");
+        segs[i].hout = hpsi_ptrs[i];
+    }
+    if (nk == 0 || nb == 0) return kOk;
+    const bool acc = !(flags & EFFM_F_OVERWRITE);
+    const bool graph = (flags & EFFM_F_GRAPH) != 0;
+    return prec == EFFM_PREC_FP64 ? apply_hd<double>(c, segs.data(), nk, nb, acc, graph, prec)
+                                  : apply_hd<float>(c, segs.data(), nk, nb, acc, graph, prec);
 }
 
 int effm_autotune(effm_ctx* h, int ikpt, int nb, int prec, int* pencil_bands, int* gemm_wgs) {

@@ -209,6 +209,7 @@ struct ApplySeg {
     int64_t ld;
     void* out[3];         // H|psi>: out[0]; dH/dk: x, y, z (NULL = skip)
     int64_t ldo;
+    void* hout = nullptr; // dH/dk together with H|psi>: the H|psi> output
 };
 
 // ---- launchers (k_local.hip) --------------------------------------------
@@ -229,9 +230,12 @@ template <typename T>
 hipError_t launch_vnl(Ctx& c, const ApplySeg* segs, int nseg, int nb, void* partial,
                       size_t partial_bytes, void* cbuf);
 
+// with_h: also the nonlocal part of H|psi> into each segment's hout (added
+// to what is there), from the <p|psi> block of the same projection
 template <typename T>
 hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accumulate,
-                       void* partial, size_t partial_bytes, void* cbuf);
+                       void* partial, size_t partial_bytes, void* cbuf,
+                       bool with_h = false);
 
 // split-K partial sums of nseg segments with nrows projector rows each
 template <typename T>

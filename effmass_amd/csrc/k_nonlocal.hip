@@ -758,7 +758,7 @@ hipError_t launch_vnl(Ctx& c, const ApplySeg* segs, int nseg, int nb, void* part
 
 template <typename T>
 hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accumulate, void* partial,
-                       size_t partial_bytes, void* cbuf) {
+                       size_t partial_bytes, void* cbuf, bool with_h) {
     using C = cx_t<T>;
     const int nch = c.nch();
     const bool f64 = sizeof(T) == 8;
@@ -801,6 +801,14 @@ hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accu
     timer_end(c, "proj_reduce");
     if (e != hipSuccess) return e;
     timer_begin(c);
+    if (with_h) {
+        // hpsi += sum_c p_c lambda_c <p_c|psi>, the first block of the projection
+        for (int i = 0; i < nseg; ++i)
+            ap[i] = ApplySegK<T>{P[i], cb + (size_t)i * Mtot * nb, static_cast<C*>(segs[i].hout), nullptr,
+                                 nullptr, nullptr, nullptr, (int64_t)segs[i].k->npw, (int64_t)nb, segs[i].ldo, 0,
+                                 segs[i].k->npw, 0};
+        e = gemm_apply<T>(c, nseg, ap, nb, nch, true);
+    }
     for (int a = 0; a < 3 && e == hipSuccess; ++a) {
         // out_a (+)= K_a psi + sum_c p_c lambda_c <dp_a,c|psi> + sum_c dp_a,c lambda_c <p_c|psi>:
         // one launch, the two projector products concatenated along k
@@ -877,8 +885,8 @@ template hipError_t launch_overlap<float>(Ctx&, int, int, int, const void*, int6
 
 template hipError_t launch_vnl<double>(Ctx&, const ApplySeg*, int, int, void*, size_t, void*);
 template hipError_t launch_vnl<float>(Ctx&, const ApplySeg*, int, int, void*, size_t, void*);
-template hipError_t launch_dhdk<double>(Ctx&, const ApplySeg*, int, int, bool, void*, size_t, void*);
-template hipError_t launch_dhdk<float>(Ctx&, const ApplySeg*, int, int, bool, void*, size_t, void*);
+template hipError_t launch_dhdk<double>(Ctx&, const ApplySeg*, int, int, bool, void*, size_t, void*, bool);
+template hipError_t launch_dhdk<float>(Ctx&, const ApplySeg*, int, int, bool, void*, size_t, void*, bool);
 
 // ---------------------------------------------------------------------------
 //  FP64 Rayleigh quotients

@@ -238,6 +238,19 @@ class Hamiltonian:
                    self._ctx, "effm_dhdk_multi")
         return outs
 
+    def apply_with_gradient_multi(self, ikpts, psis, hpsis, outs: "GradHPsi", prec: str = "fp64",
+                                  overwrite: bool = False, graph: bool = False):
+        """H|psi> into hpsis and dH/dk|psi> into outs for several k-points,
+        with one projection GEMM for both (the k.p consumer's pair); same
+        results as apply_multi followed by gradient_multi."""
+        p = PREC[prec]
+        nk, ks, nb, pp, lds, (hp, ox, oy, oz) = self._batch(ikpts, psis, [hpsis, outs.x, outs.y, outs.z], p)
+        self._sync_stream()
+        flags = (_lib.EFFM_F_OVERWRITE if overwrite else 0) | (_lib.EFFM_F_GRAPH if graph else 0)
+        _lib.check(self._lib.effm_hpsi_dhdk_multi(self._ctx, nk, ks, nb, pp, lds, hp, ox, oy, oz, p, flags),
+                   self._ctx, "effm_hpsi_dhdk_multi")
+        return hpsis, outs
+
     def overlap(self, A, B):
         """C = conj(A) @ B.T for device tensors A [m, k], B [n, k] (complex128
         -> FP64 path, complex64 -> mixed), split-K sums added in FP64; C has

@@ -149,6 +149,19 @@ int effm_dhdk_multi(effm_ctx* ctx, int nk, const int* ikpts, int nb,
                     void* const* out_x, void* const* out_y, void* const* out_z,
                     int prec, int flags);
 
+/* H|psi> into hpsi_ptrs[i] and dH/dk_{x,y,z}|psi> into out_x/y/z[i] for the
+ * same batch, as effm_hpsi_multi followed by effm_dhdk_multi with the same
+ * flags (NULL direction arrays skip a direction), but with one projection
+ * GEMM: the nonlocal part of H|psi> takes the <p|psi> block of the
+ * gradient's <p, dp_x, dp_y, dp_z | psi> projection.  The k.p consumer needs
+ * both at every k-point.  No reference counterpart; results equal the two
+ * separate calls. */
+int effm_hpsi_dhdk_multi(effm_ctx* ctx, int nk, const int* ikpts, int nb,
+                         const void* const* psi_ptrs, const int64_t* ld,
+                         void* const* hpsi_ptrs, void* const* out_x,
+                         void* const* out_y, void* const* out_z, int prec,
+                         int flags);
+
 /* FP64 Rayleigh quotients E_b = Re<psi_b|hpsi_b> / <psi_b|psi_b> over npw
  * elements (test_hpsi_eigen.cpp:49-56 computes this on the host); E is a
  * device pointer unless EFFM_F_HOST.  prec selects the element type. */

@@ -136,6 +136,55 @@ def test_dhdk_multi_equals_single_k():
     H.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_fused_hpsi_dhdk_equals_separate_calls(prec):
+    """effm_hpsi_dhdk_multi (one projection GEMM for H|psi> and dH/dk|psi>)
+    against effm_hpsi_multi + effm_dhdk_multi: overwrite, then accumulate,
+    then a graph replay; FP64 also against the oracle.  Tolerance: FP64
+    1e-13 relative per band (the split-K sums of the fused projection are
+    grouped like the gradient's), mixed 1e-6."""
+    torch = _torch()
+    wl = _small_multi(5, nb=8, seed=23)
+    H = _load(wl)
+    cdt = torch.complex128 if prec == "fp64" else torch.complex64
+    tol = 1e-13 if prec == "fp64" else 1e-6
+    ks = list(range(5))
+    psis = [torch.from_numpy(p).cuda().to(cdt) for p in wl.psi]
+    hs = [torch.zeros_like(p) for p in psis]
+    g = GradHPsi([torch.zeros_like(p) for p in psis], None, [torch.zeros_like(p) for p in psis])
+    rh = [torch.zeros_like(p) for p in psis]
+    rg = GradHPsi([torch.zeros_like(p) for p in psis], None, [torch.zeros_like(p) for p in psis])
+
+    def check(tag):
+        torch.cuda.synchronize()
+        for ik in ks:
+            for got, ref in ((hs[ik], rh[ik]), (g.x[ik], rg.x[ik]), (g.z[ik], rg.z[ik])):
+                err = band_rel(got.cpu().numpy().astype(np.complex128), ref.cpu().numpy().astype(np.complex128))
+                assert err <= tol, (tag, ik, err)
+        log(f"{prec} {tag}: fused == separate")
+
+    H.apply_with_gradient_multi(ks, psis, hs, g, prec=prec, overwrite=True)
+    H.apply_multi(ks, psis, rh, prec=prec, overwrite=True)
+    H.gradient_multi(ks, psis, rg, prec=prec, overwrite=True)
+    check("overwrite")
+    H.apply_with_gradient_multi(ks, psis, hs, g, prec=prec)
+    H.apply_multi(ks, psis, rh, prec=prec)
+    H.gradient_multi(ks, psis, rg, prec=prec)
+    check("accumulate")
+    for _ in range(3):
+        H.apply_with_gradient_multi(ks, psis, hs, g, prec=prec, overwrite=True, graph=True)
+    H.apply_multi(ks, psis, rh, prec=prec, overwrite=True)
+    H.gradient_multi(ks, psis, rg, prec=prec, overwrite=True)
+    check("graph replay")
+    if prec == "fp64":
+        hp = oracle.hpsi(wl.cell, wl.kpts[3], wl.psi[3])
+        gx, gy, gz = oracle.dhdk(wl.cell, wl.kpts[3], wl.psi[3])
+        assert band_rel(hs[3].cpu().numpy(), hp) <= 1e-12
+        assert band_rel(g.x[3].cpu().numpy(), gx) <= 1e-12
+        assert band_rel(g.z[3].cpu().numpy(), gz) <= 1e-12
+    H.close()
+
+
 def test_graph_replay_matches_eager():
     """Captured apply sequences replayed several times, with eager work and
     new psi contents in between; every replay equals the eager result."""
