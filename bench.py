@@ -23,7 +23,7 @@ MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 RY_PER_HA = 2.0
 
 
-TRAFFIC_PROFILE = "profiles/hpsi_config1_traffic.md"
+TRAFFIC_PROFILE = "profiles/hpsi_si64_traffic.md"
 
 
 def measured_traffic(kernel, prec):
@@ -68,7 +68,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--serve-queries", type=int, default=1000,
-                    help="config-4 effective-mass queries replayed at 50/s (0 = skip)")
+                    help="effective-mass queries of the serving stream replayed at 50/s (0 = skip)")
     ap.add_argument("--serve-light", type=int, default=100,
                     help="queries of a second stream at --serve-light-rate, below the server's "
                          "capacity, for the unqueued latency (0 = skip)")
@@ -76,7 +76,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
                     help="time one precision only (profiling runs)")
-    ap.add_argument("--config2-steps", type=int, default=3,
+    ap.add_argument("--multik-steps", type=int, default=3,
                     help="timed steps of the 32-k-point batched H|psi> + dH/dk leg (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="run a plain device copy of MB megabytes 5 times and exit "
@@ -103,7 +103,7 @@ def main():
     from effmass_amd.hamiltonian import Hamiltonian
 
     t_setup = time.time()
-    wl = synth.build(1)
+    wl = synth.build("si64_cubic")
     cell, kp, psi_h = wl.cell, wl.kpts[0], wl.psi[0]
     nb, npw = psi_h.shape
     H = Hamiltonian(cell, device=0)
@@ -216,7 +216,7 @@ def main():
 
     H.close()
     # 32 k-points of the 64-atom cell in batched launches, graph replays
-    config2 = config2_leg(args.config2_steps) if args.config2_steps > 0 else None
+    multik = multik_leg(args.multik_steps) if args.multik_steps > 0 else None
 
     # serving: Poisson(50/s) effective-mass queries over 2-64 atom cells
     serving_stats = None
@@ -250,7 +250,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 FFT/GEMM, fp64 D contraction + Rayleigh",
-        "data": "synthetic (seeded SplitMix64, effmass_amd/synth.py config 1)",
+        "data": "synthetic (seeded SplitMix64, effmass_amd/synth.py workload si64_cubic)",
         "config": {"workload": "Si64 2x2x2 diamond supercell, a=20.52 bohr, ecut 40 Ry, k=(0.1,0,0), "
                                "npw 36945, FFT 90^3, 256 bands, 512 projectors; step = H|psi> on all bands",
                    "bands": nb, "npw": npw, "grid": list(cell.n), "projectors": cell.nchannels()},
@@ -263,7 +263,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
         "tuned_launch_shapes": tuned,
-        "config2": config2,
+        "multi_k": multik,
         "m_star_latency": _latency_summary(serving_stats),
         "serving": serving_stats,
         "setup_s": setup_s,
@@ -271,7 +271,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def config2_leg(steps, warmup=1):
+def multik_leg(steps, warmup=1):
     """H|psi> and dH/dk_{x,y,z}|psi> for 32 k-points (4x4x2 grid, ragged npw
     36,816-37,073) x 128 bands of the 64-atom cell: effm_hpsi_dhdk_multi
     (one projection GEMM for both, up to 16 k-points per launch), replayed
@@ -281,7 +281,7 @@ def config2_leg(steps, warmup=1):
     import torch
     from effmass_amd import synth
     from effmass_amd.hamiltonian import GradHPsi, Hamiltonian
-    wl = synth.build(2, with_psi=False)
+    wl = synth.build("si64_kgrid", with_psi=False)
     H = Hamiltonian(wl.cell, device=0)
     for ik, kp in enumerate(wl.kpts):
         H.load_kpoint(ik, kp)

@@ -1,4 +1,4 @@
-"""Serving loop for effective-mass queries (the config-4 workload).
+"""Serving loop for effective-mass queries (a stream of 2-64 atom cells).
 
 A query is a cell of 2-64 atoms (diamond-derived, 135 bohr^3 per atom), a
 cutoff, a band count and 1-4 k-points; its answer is the k.p effective-mass
@@ -172,7 +172,7 @@ def run_query(q: MassQuery, prec="mixed", device=0, wl=None, budget=None) -> Mas
             op = H.at_k(ik)
             # Chebyshev-filtered subspace iteration: H applies are cheap next
             # to the dense eigenproblems (measured 1.3-5x faster than LOBPCG
-            # on the config-4 queries, same eigenvalues to 1e-7 Ha)
+            # on the serving query mix, same eigenvalues to 1e-7 Ha)
             res = eigensolver.chfsi(lambda V, HV: op(V, HV, prec=prec, overwrite=True), X0, kin,
                                     nconv=min(nb, band + 2), tol=tol, maxiter=200, work_dtype=wdt,
                                     gram=H.overlap, degree=20, eigh=eigensolver.eigh_host)

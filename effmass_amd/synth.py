@@ -1,7 +1,7 @@
 """Seeded synthetic inputs for the H|psi> path.
 
-The five workloads (cells, cutoffs, potentials, projectors, wavefunctions) are
-generated deterministically from (config, seed) with a SplitMix64 stream and
+The workloads (cells, cutoffs, potentials, projectors, wavefunctions) are
+generated deterministically from (name, seed) with a SplitMix64 stream and
 Box-Muller normals, so the device path and the CPU oracle see identical bytes.
 
 Conventions follow the reference:
@@ -293,7 +293,13 @@ def make_psi(kp: KPoint, nb: int, rng: SplitMix64) -> np.ndarray:
 # --------------------------------------------------------------------------
 #  The five workloads
 # --------------------------------------------------------------------------
-CONFIG_SEEDS = {0: 0xEFF0, 1: 0xEFF1, 2: 0xEFF2, 3: 0xEFF3, 4: 0xEFF4}
+# The named workloads: cell, grid, cutoff, bands, k-points.
+#   si2_fcc     2-atom fcc Si, a = 10.26 bohr, 30 Ry, 36^3, 8 bands, Gamma
+#   si64_cubic  64-atom diamond supercell, a = 20.52 bohr, 40 Ry, 90^3, 256
+#               bands, k = (0.1, 0, 0)
+#   si64_kgrid  the same cell, 4x4x2 k-grid (32 k-points), 128 bands
+#   si8_cubic   8-atom cubic Si, a = 10.26 bohr, 50 Ry, 48^3, 200 bands, Gamma
+WORKLOAD_SEEDS = {"si2_fcc": 0xEFF0, "si64_cubic": 0xEFF1, "si64_kgrid": 0xEFF2, "si8_cubic": 0xEFF3}
 
 
 def mp_grid(m1, m2, m3):
@@ -307,26 +313,27 @@ def mp_grid(m1, m2, m3):
     return ks
 
 
-def build(config: int, nb: int | None = None, nk: int | None = None, seed: int | None = None,
+def build(name: str, nb: int | None = None, nk: int | None = None, seed: int | None = None,
           with_psi: bool = True, two_species: bool = False) -> Workload:
-    """Build workload `config` (0..3).  nb / nk shrink the band or k-point
-    count for quick parity cases; the cell, grid and cutoff never change."""
-    seed = CONFIG_SEEDS[config] if seed is None else seed
+    """Build the named workload (WORKLOAD_SEEDS).  nb / nk shrink the band
+    or k-point count for quick parity cases; the cell, grid and cutoff never
+    change."""
+    if name not in WORKLOAD_SEEDS:
+        raise ValueError(f"unknown workload {name!r}")
+    seed = WORKLOAD_SEEDS[name] if seed is None else seed
     rng = SplitMix64(seed)
-    if config == 0:
+    if name == "si2_fcc":
         A, tau = fcc_primitive(10.26)
         n, ecut, nbd, ks = (36, 36, 36), 30.0, 8, [np.zeros(3)]
-    elif config == 1:
+    elif name == "si64_cubic":
         A, tau = diamond_cubic(10.26, rep=2)
         n, ecut, nbd, ks = (90, 90, 90), 40.0, 256, [np.array([0.1, 0.0, 0.0])]
-    elif config == 2:
+    elif name == "si64_kgrid":
         A, tau = diamond_cubic(10.26, rep=2)
         n, ecut, nbd, ks = (90, 90, 90), 40.0, 128, mp_grid(4, 4, 2)
-    elif config == 3:
+    else:
         A, tau = diamond_cubic(10.26, rep=1)
         n, ecut, nbd, ks = (48, 48, 48), 50.0, 200, [np.zeros(3)]
-    else:
-        raise ValueError(f"unknown config {config}")
     if nk is not None:
         ks = ks[:nk]
     nbands = nbd if nb is None else nb
@@ -340,7 +347,7 @@ def build(config: int, nb: int | None = None, nk: int | None = None, seed: int |
     cell = Cell(A=A, n=n, vloc=vloc, tau=np.ascontiguousarray(tau, dtype=np.float64),
                 species=species, species_list=species_list)
     kpts = [g_sphere(A, n, k, ecut) for k in ks]
-    wl = Workload(name=f"config{config}", cell=cell, kpts=kpts, nb=nbands, ecut_ry=ecut, seed=seed)
+    wl = Workload(name=name, cell=cell, kpts=kpts, nb=nbands, ecut_ry=ecut, seed=seed)
     if with_psi:
         wl.psi = [make_psi(kp, nbands, rng) for kp in kpts]
     return wl

@@ -16,10 +16,10 @@ def pytest_configure(config):
 _WL = {}
 
 
-def workload(config, **kw):
+def workload(name, **kw):
     """Cached synthetic workloads (effmass_amd.synth.build)."""
     from effmass_amd import synth
-    key = (config, tuple(sorted(kw.items())))
+    key = (name, tuple(sorted(kw.items())))
     if key not in _WL:
-        _WL[key] = synth.build(config, **kw)
+        _WL[key] = synth.build(name, **kw)
     return _WL[key]

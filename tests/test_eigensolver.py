@@ -28,7 +28,7 @@ def test_lobpcg_random_hermitian():
 
 
 def test_lobpcg_dense_fcc_h():
-    wl = workload(0)
+    wl = workload("si2_fcc")
     kp = wl.kpts[0]
     Hd = refmath.dense_h(wl.cell, kp)
     H = torch.from_numpy(Hd)
@@ -57,7 +57,7 @@ def test_chfsi_random_hermitian_and_fcc():
     assert np.abs(res.E[:10].numpy() - ref).max() < 1e-12 * max(1, abs(ref).max())
     G = (res.X.conj() @ res.X.T).numpy()
     assert np.abs(G - np.eye(14)).max() < 1e-12
-    wl = workload(0)
+    wl = workload("si2_fcc")
     kp = wl.kpts[0]
     Hf = refmath.dense_h(wl.cell, kp)
     Ht = torch.from_numpy(Hf)

@@ -49,13 +49,13 @@ def _load(wl):
     return H
 
 
-def test_multi_config2_shapes_equals_single_k():
+def test_multi_kgrid_shapes_equals_single_k():
     """64-atom cell, 5 k-points of the 4x4x2 grid (ragged npw), 8 bands: the
     batched launch against one effm_hpsi per k-point and the oracle."""
     torch = _torch()
-    wl = workload(2, nb=8, nk=5)
+    wl = workload("si64_kgrid", nb=8, nk=5)
     assert len({kp.npw for kp in wl.kpts}) > 1
-    log(f"config-2 shapes npw {[kp.npw for kp in wl.kpts]}")
+    log(f"si64 k-grid shapes npw {[kp.npw for kp in wl.kpts]}")
     H = _load(wl)
     psis = [torch.from_numpy(p).cuda() for p in wl.psi]
     outs = [torch.zeros_like(p) for p in psis]
@@ -240,7 +240,7 @@ def test_autotune_choices_keep_parity():
     """The autotuner picks among the candidate shapes; every later apply,
     whatever was picked, still matches the oracle."""
     torch = _torch()
-    wl = workload(1, nb=24)
+    wl = workload("si64_cubic", nb=24)
     H = Hamiltonian(wl.cell)
     H.load_kpoint(0, wl.kpts[0])
     for prec in ("fp64", "mixed"):

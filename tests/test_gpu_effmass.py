@@ -1,4 +1,4 @@
-"""Effective masses on the device (config 3: 8-atom cubic cell, ecut 50 Ry,
+"""Effective masses on the device (the 8-atom cubic cell, ecut 50 Ry,
 npw 6451, 200 bands at Gamma, bands n = 4, 5):
 * FP64 k.p masses equal the oracle's k.p masses on the same eigenpairs;
 * the mixed path (FP32 H|psi>, dH/dk; FP64 sums) is within 1e-3 relative;
@@ -22,7 +22,7 @@ _CACHE = {}
 def solve(prec):
     if prec in _CACHE:
         return _CACHE[prec]
-    wl = workload(3, nb=2)
+    wl = workload("si8_cubic", nb=2)
     cell, kp = wl.cell, wl.kpts[0]
     H = Hamiltonian(cell)
     H.load_kpoint(0, kp)

@@ -54,12 +54,12 @@ def log(msg):
     sys.stdout.flush()
 
 
-def test_config0_fp64():
-    wl = workload(0)
+def test_si2_fcc_fp64():
+    wl = workload("si2_fcc")
     got = run_h(wl)
     ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0])
     e = band_rel(got, ref)
-    log(f"config0 fp64 rel err {e:.2e}")
+    log(f"si2 fcc fp64 rel err {e:.2e}")
     assert e <= 1e-12
 
 
@@ -75,26 +75,26 @@ def test_two_species_offgamma_fp64():
     assert band_rel(got, ref) <= 1e-12
 
 
-def test_config3_fp64_subset():
-    wl = workload(3, nb=24)
+def test_si8_cubic_fp64_subset():
+    wl = workload("si8_cubic", nb=24)
     got = run_h(wl)
     ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0], nthreads=8)
     e = band_rel(got, ref)
-    log(f"config3 fp64 rel err {e:.2e}")
+    log(f"si8 cubic fp64 rel err {e:.2e}")
     assert e <= 1e-12
 
 
-def test_config1_fp64_subset():
-    wl = workload(1, nb=16)
+def test_si64_cubic_fp64_subset():
+    wl = workload("si64_cubic", nb=16)
     got = run_h(wl)
     ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0], nthreads=8)
     e = band_rel(got, ref)
-    log(f"config1 fp64 rel err {e:.2e}")
+    log(f"si64 cubic fp64 rel err {e:.2e}")
     assert e <= 1e-12
 
 
-def test_config1_mixed_energies():
-    wl = workload(1, nb=16)
+def test_si64_cubic_mixed_energies():
+    wl = workload("si64_cubic", nb=16)
     got = run_h(wl, prec="mixed")
     ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0], nthreads=8)
     psi = wl.psi[0]
@@ -102,14 +102,14 @@ def test_config1_mixed_energies():
     p32 = psi.astype(np.complex64).astype(np.complex128)
     e_mix = np.real(np.sum(p32.conj() * got, axis=1)) / np.sum(np.abs(p32) ** 2, axis=1)
     de = np.abs(e_mix - e_ref).max()
-    log(f"config1 mixed max |dE| {de:.2e} Ha, rel hpsi err {band_rel(got, ref):.2e}")
+    log(f"si64 cubic mixed max |dE| {de:.2e} Ha, rel hpsi err {band_rel(got, ref):.2e}")
     assert de <= RY_TOL_HA
     assert band_rel(got, ref) <= 2e-5
 
 
 def test_accumulate_overwrite_and_host_path():
     torch = _torch()
-    wl = workload(0)
+    wl = workload("si2_fcc")
     H = Hamiltonian(wl.cell)
     H.load_kpoint(0, wl.kpts[0])
     psi = torch.from_numpy(wl.psi[0]).cuda()
@@ -160,7 +160,7 @@ def test_dhdk_fp64():
 
 def test_dhdk_mixed():
     torch = _torch()
-    wl = workload(0)
+    wl = workload("si2_fcc")
     H = Hamiltonian(wl.cell)
     H.load_kpoint(0, wl.kpts[0])
     psi = torch.from_numpy(wl.psi[0].astype(np.complex64)).cuda()
@@ -175,7 +175,7 @@ def test_dhdk_mixed():
 
 def test_rayleigh_fp64_and_mixed():
     torch = _torch()
-    wl = workload(0)
+    wl = workload("si2_fcc")
     H = Hamiltonian(wl.cell)
     H.load_kpoint(0, wl.kpts[0])
     ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0])
@@ -190,7 +190,7 @@ def test_rayleigh_fp64_and_mixed():
 
 def test_error_paths():
     torch = _torch()
-    wl = workload(0)
+    wl = workload("si2_fcc")
     H = Hamiltonian(wl.cell)
     with pytest.raises(_lib.EffmOutOfRange):
         H.at_k(0)                                           # no k-point loaded
@@ -233,10 +233,10 @@ def test_error_paths():
             Hs.close()
 
 
-def test_config1_full_hermitian_and_linear():
+def test_si64_cubic_full_hermitian_and_linear():
     """All 256 bands at the 64-atom size: size-independent properties."""
     torch = _torch()
-    wl = workload(1)
+    wl = workload("si64_cubic")
     H = Hamiltonian(wl.cell)
     H.load_kpoint(0, wl.kpts[0])
     psi = torch.from_numpy(wl.psi[0]).cuda()
@@ -244,7 +244,7 @@ def test_config1_full_hermitian_and_linear():
     H.at_k(0)(psi, h)
     M = (psi.conj() @ h.T).cpu().numpy()                     # <psi_m|H|psi_n>
     herm = np.abs(M - M.conj().T).max() / np.abs(M).max()
-    log(f"config1 hermiticity {herm:.2e}")
+    log(f"si64 cubic hermiticity {herm:.2e}")
     assert herm <= 1e-12
     a, b = 0.3 - 0.7j, -1.1 + 0.2j
     comb = (a * psi[:8] + b * psi[8:16]).contiguous()
@@ -259,14 +259,14 @@ def test_config1_full_hermitian_and_linear():
     E64 = H.rayleigh(psi, h).cpu().numpy()
     E32 = H.rayleigh(p32, h32, prec="mixed").cpu().numpy()
     de = np.abs(E32 - E64).max()
-    log(f"config1 all-band mixed |dE| max {de:.2e} Ha")
+    log(f"si64 cubic all-band mixed |dE| max {de:.2e} Ha")
     assert de <= RY_TOL_HA
     H.close()
 
 
 def test_multi_k_ragged():
     torch = _torch()
-    wl = workload(2, nb=4, nk=3)
+    wl = workload("si64_kgrid", nb=4, nk=3)
     H = Hamiltonian(wl.cell)
     for ik, kp in enumerate(wl.kpts):
         H.load_kpoint(ik, kp)

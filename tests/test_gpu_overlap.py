@@ -22,7 +22,7 @@ def _torch():
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (37, 101, 777), (256, 242, 24001), (64, 64, 0), (130, 3, 5000)])
 def test_overlap_matches_matmul(m, n, k):
     torch = _torch()
-    wl = workload(0)
+    wl = workload("si2_fcc")
     H = Hamiltonian(wl.cell)
     g = torch.Generator(device="cuda").manual_seed(m * 1000 + n + k)
     A = torch.randn(m, k, dtype=torch.complex128, device="cuda", generator=g)
@@ -40,7 +40,7 @@ def test_device_lobpcg_with_engine_overlaps():
     """LOBPCG on the device (H|psi> and overlaps through the engine) on the
     2-atom cell, against dense diagonalisation of H_GG'."""
     torch = _torch()
-    wl = workload(0)
+    wl = workload("si2_fcc")
     kp = wl.kpts[0]
     ref = np.linalg.eigvalsh(refmath.dense_h(wl.cell, kp))[:8]
     H = Hamiltonian(wl.cell)

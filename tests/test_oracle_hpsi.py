@@ -16,13 +16,13 @@ def _rel(a, b):
     return np.abs(a - b).max() / np.abs(b).max()
 
 
-def test_npw_of_configs():
-    assert workload(0, nb=2).kpts[0].npw == 725
-    assert workload(3, nb=2).kpts[0].npw == 6451
+def test_npw_of_workloads():
+    assert workload("si2_fcc", nb=2).kpts[0].npw == 725
+    assert workload("si8_cubic", nb=2).kpts[0].npw == 6451
 
 
 def test_local_part_vs_dense():
-    wl = workload(0)
+    wl = workload("si2_fcc")
     cell = copy.copy(wl.cell)
     sp = copy.copy(cell.species_list[0]); sp.lam = np.zeros_like(sp.lam)
     cell.species_list = [sp]                      # |lambda| < 1e-15: no projectors
@@ -33,7 +33,7 @@ def test_local_part_vs_dense():
 
 
 def test_full_vs_dense():
-    wl = workload(0)
+    wl = workload("si2_fcc")
     kp, psi = wl.kpts[0], wl.psi[0]
     H = refmath.dense_h(wl.cell, kp)
     got = oracle.hpsi(wl.cell, kp, psi)
@@ -45,7 +45,7 @@ def test_full_vs_dense():
 
 
 def test_two_species_vs_dense():
-    wl = workload(0, two_species=True)
+    wl = workload("si2_fcc", two_species=True)
     kp, psi = wl.kpts[0], wl.psi[0]
     H = refmath.dense_h(wl.cell, kp)
     assert _rel(oracle.hpsi(wl.cell, kp, psi), psi @ H.T) < 1e-8
@@ -60,7 +60,7 @@ def test_offgamma_vs_dense():
 
 
 def test_hermitian_and_accumulate():
-    wl = workload(0)
+    wl = workload("si2_fcc")
     kp, psi = wl.kpts[0], wl.psi[0]
     h = oracle.hpsi(wl.cell, kp, psi)
     M = psi.conj() @ h.T                        # <psi_m|H|psi_n>
@@ -70,7 +70,7 @@ def test_hermitian_and_accumulate():
 
 
 def test_constant_potential_shift():
-    wl = workload(0)
+    wl = workload("si2_fcc")
     cell = copy.copy(wl.cell)
     cell.vloc = np.full_like(cell.vloc, -0.3)
     sp = copy.copy(cell.species_list[0]); sp.lam = np.zeros_like(sp.lam)
@@ -104,7 +104,7 @@ def test_dhdk_vs_finite_difference():
 
 
 def test_dhdk_vs_dense_derivative():
-    wl = workload(0)
+    wl = workload("si2_fcc")
     cell, kp, psi = wl.cell, wl.kpts[0], wl.psi[0]
     P, lam, dP = refmath.projectors(cell, kp, deriv=True)
     g = oracle.dhdk(cell, kp, psi)
@@ -115,7 +115,7 @@ def test_dhdk_vs_dense_derivative():
 
 
 def test_dhdk_kinetic_only_and_skip():
-    wl = workload(0)
+    wl = workload("si2_fcc")
     cell = copy.copy(wl.cell)
     sp = copy.copy(cell.species_list[0]); sp.lam = np.zeros_like(sp.lam)
     cell.species_list = [sp]

@@ -1,4 +1,4 @@
-"""Host-side parts of the serving loop: the config-4 query stream, cells
+"""Host-side parts of the serving loop: the serving query stream, cells
 with 135 bohr^3 per atom, FFT grids, footprint estimates."""
 import numpy as np
 
