@@ -93,9 +93,9 @@ def main():
         print(json.dumps({"copy_probe_bytes": 8 * n, "launches": 5}), flush=True)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the H|psi> path shards over "
-                                   "k-points and bands without collectives (see DESIGN.md), "
-                                   "multi-GPU runs are not part of it"}))
+        print(json.dumps({"error": "bench.py measures one GPU (--gpus 1); the H|psi> path shards over "
+                                   "k-points and bands without collectives, and multi-GPU runs are not "
+                                   "implemented"}))
         sys.exit(2)
 
     import torch

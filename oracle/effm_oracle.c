@@ -1,7 +1,7 @@
 /*
  * effm_oracle.c — FP64 CPU restatement of the effmass H|psi> hot path.
  *
- * TEST INFRASTRUCTURE ONLY: the checker used by tests/, smoke() and the
+ * TEST INFRASTRUCTURE ONLY: the oracle used by tests/, smoke() and the
  * cpu_baseline leg of bench.py.  Nothing in the product links it.
  *
  * Parity pins (each exercised by tests/test_oracle_*.py):

@@ -1,7 +1,7 @@
 /*
  * effm_oracle.h — FP64 CPU restatement of the effmass H|psi> hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is linked or
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle: it is linked or
  * loaded only by tests/, by __graft_entry__.smoke() and by bench.py's
  * cpu_baseline leg.  The product path (libeffm_hip.so) never calls it.
  *

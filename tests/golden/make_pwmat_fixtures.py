@@ -1,7 +1,7 @@
 """Copies the PWmat / UPF data files the reference's own tests read into
 tests/golden/pwmat/, trimming OUT.GKK files to their first k-point (same
 Fortran record layout, nkpt = 1).  Run in a checkout that has the reference
-at REF (default /root/reference); the GPU box only sees the copies.
+at REF (default /root/reference); GPU runs only see the copies.
 
 Sources (reference test data): test/data_nonscf/{OUT2.GKK, IN.VR, OUT2.EIGEN,
 atom.config, Si.SG15.PBE.UPF}, test/data_scf/{OUT.GKK, OUT.VR, OUT.EIGEN,
