@@ -26,6 +26,7 @@ returns the same numbers alone and in the mix.
 from __future__ import annotations
 
 import collections
+import dataclasses
 import math
 import multiprocessing as mp
 import threading
@@ -90,6 +91,7 @@ class MassQuery:
     seed: int
     arrival: float = 0.0          # seconds after the start of the run
     band: int = -1                # band index (default nb // 2)
+    k0: int = 0                   # index of kfracs[0] in the query this one is a part of
 
 
 @dataclass
@@ -102,6 +104,7 @@ class MassAnswer:
     finish: float
     npw: list = field(default_factory=list)
     error: str | None = None
+    k0: int = 0                   # first k-point of the query part this answers
 
 
 def make_stream(n, rate=50.0, seed=2024, nb_range=(16, 256), sizes=SIZES, ecuts=ECUTS, max_k=4):
@@ -168,7 +171,7 @@ def run_query(q: MassQuery, prec="mixed", device=0, wl=None, budget=None) -> Mas
             H.load_kpoint(ik, kp)
             kin = torch.from_numpy(kp.kin).to(f"cuda:{device}")
             nb = min(q.nb, kp.npw)
-            X0 = eigensolver.random_start(nb, kp.npw, kin, seed=q.seed + ik, device=f"cuda:{device}")
+            X0 = eigensolver.random_start(nb, kp.npw, kin, seed=q.seed + q.k0 + ik, device=f"cuda:{device}")
             op = H.at_k(ik)
             # Chebyshev-filtered subspace iteration: H applies are cheap next
             # to the dense eigenproblems (measured 1.3-5x faster than LOBPCG
@@ -185,7 +188,8 @@ def run_query(q: MassQuery, prec="mixed", device=0, wl=None, budget=None) -> Mas
     finally:
         H.close()
     t1 = time.perf_counter()
-    return MassAnswer(qid=q.qid, inv_mass=inv, energies=en, latency=t1 - t0, start=t0, finish=t1, npw=npws)
+    return MassAnswer(qid=q.qid, inv_mass=inv, energies=en, latency=t1 - t0, start=t0, finish=t1, npw=npws,
+                      k0=q.k0)
 
 
 def _worker_main(slot, device, prec, tasks, results, host_threads=2):
@@ -215,7 +219,7 @@ def _worker_main(slot, device, prec, tasks, results, host_threads=2):
         except Exception as e:                                   # surfaced in the answer
             now = time.perf_counter()
             ans = MassAnswer(qid=q.qid, inv_mass=[], energies=[], latency=now - t_arr, start=t_arr,
-                             finish=now, error=f"{type(e).__name__}: {e}")
+                             finish=now, error=f"{type(e).__name__}: {e}", k0=q.k0)
         # torch keeps the query's blocks cached for the next query of this
         # slot (no device-wide free between queries)
         results.put(("done", slot, ans))
@@ -224,17 +228,23 @@ def _worker_main(slot, device, prec, tasks, results, host_threads=2):
 class MassServer:
     """Admits queries by HBM footprint into at most `slots` concurrent
     worker processes (at most 15, next to the caller's own process).
+    A query with several k-points is admitted as one part per k-point (the
+    k-points are independent eigenproblems on the same cell), so idle slots
+    shorten its latency; the parts' results are merged in k order.
     Timestamps are time.perf_counter() (CLOCK_MONOTONIC, the same clock in
     every process)."""
 
     MAX_SLOTS = 15
 
-    def __init__(self, device=0, hbm_budget=None, slots=4, prec="mixed", start_timeout=600.0):
+    def __init__(self, device=0, hbm_budget=None, slots=4, prec="mixed", start_timeout=600.0,
+                 split_kpoints=True):
         if not 1 <= slots <= self.MAX_SLOTS:
             raise ValueError(f"slots must be 1..{self.MAX_SLOTS}")
         self.device = device
         self.prec = prec
         self.slots = slots
+        self.split_kpoints = split_kpoints
+        self._parts = {}                  # qid -> [parts expected, {k0: answer}, arrival]
         ctx = mp.get_context("spawn")
         self._tasks = [ctx.Queue() for _ in range(slots)]
         self._results = ctx.Queue()
@@ -285,8 +295,39 @@ class MassServer:
                                                   error="query exceeds the server's HBM budget")
             return
         with self._lock:
-            self._pending.append((q, need, t_arrival))
+            nk = len(q.kfracs)
+            if self.split_kpoints and nk > 1:
+                self._parts[q.qid] = [nk, {}, t_arrival]
+                for ik in range(nk):
+                    part = dataclasses.replace(q, kfracs=[q.kfracs[ik]], k0=q.k0 + ik)
+                    self._pending.append((part, need, t_arrival))
+            else:
+                self._pending.append((q, need, t_arrival))
             self._admit()
+
+    def _answer(self, ans: MassAnswer):
+        """Record an answer (lock held); parts of a split query are merged
+        once all of them are in."""
+        rec = self._parts.get(ans.qid)
+        if rec is None:
+            self._answers[ans.qid] = ans
+            return
+        nk, got, t_arr = rec
+        got[ans.k0] = ans
+        if len(got) < nk:
+            return
+        del self._parts[ans.qid]
+        parts = [got[k] for k in sorted(got)]
+        errs = [a.error for a in parts if a.error]
+        finish = max(a.finish for a in parts)
+        merged = MassAnswer(qid=ans.qid, inv_mass=[m for a in parts for m in a.inv_mass],
+                            energies=[e for a in parts for e in a.energies], latency=finish - t_arr,
+                            start=min(a.start for a in parts), finish=finish,
+                            npw=[n for a in parts for n in a.npw], error=errs[0] if errs else None,
+                            k0=parts[0].k0)
+        if errs:
+            merged.inv_mass, merged.energies = [], []
+        self._answers[ans.qid] = merged
 
     def _admit(self):
         # FIFO: the head waits until a slot is free and its footprint fits
@@ -315,7 +356,7 @@ class MassServer:
                 self._inflight.pop(slot, None)
                 self._running -= 1
                 self._free.append(slot)
-                self._answers[ans.qid] = ans
+                self._answer(ans)
                 self._admit()
                 self._lock.notify_all()
 
@@ -331,9 +372,9 @@ class MassServer:
                 q = self._inflight.pop(slot, None)
                 if q is not None:
                     now = time.perf_counter()
-                    self._answers[q.qid] = MassAnswer(qid=q.qid, inv_mass=[], energies=[], latency=0.0, start=now,
-                                                      finish=now, error=f"serving slot {slot} exited "
-                                                                        f"(code {p.exitcode})")
+                    self._answer(MassAnswer(qid=q.qid, inv_mass=[], energies=[], latency=0.0, start=now,
+                                            finish=now, error=f"serving slot {slot} exited (code {p.exitcode})",
+                                            k0=q.k0))
                 self._lock.notify_all()
 
     def wait(self, n, timeout=None, progress=None):
@@ -373,11 +414,12 @@ class MassServer:
             return dict(self._answers)
 
 
-def serve(queries, device=0, slots=4, prec="mixed", hbm_budget=None, realtime=True, timeout=None, progress=None):
+def serve(queries, device=0, slots=4, prec="mixed", hbm_budget=None, realtime=True, timeout=None, progress=None,
+          split_kpoints=True):
     """Replay `queries` at their arrival times (realtime) or all at once;
     return (answers by qid, stats).  progress(done, total) is called about
     once a second while waiting."""
-    srv = MassServer(device=device, hbm_budget=hbm_budget, slots=slots, prec=prec)
+    srv = MassServer(device=device, hbm_budget=hbm_budget, slots=slots, prec=prec, split_kpoints=split_kpoints)
     try:
         t0 = time.perf_counter()
         for q in sorted(queries, key=lambda x: x.arrival):
