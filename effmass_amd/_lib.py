@@ -30,7 +30,7 @@ EXPORTS = (
     "effm_ctx_create", "effm_ctx_destroy", "effm_set_stream", "effm_last_error",
     "effm_mem_info", "effm_cell_upload", "effm_kpt_upload", "effm_kpt_dim",
     "effm_kpt_release", "effm_hpsi", "effm_dhdk", "effm_hpsi_multi", "effm_dhdk_multi",
-    "effm_hpsi_dhdk_multi", "effm_rayleigh",
+    "effm_hpsi_dhdk_multi", "effm_lincomb", "effm_rayleigh",
     "effm_overlap", "effm_autotune", "effm_set_timing", "effm_get_timing", "effm_reset_timing",
 )
 
@@ -116,6 +116,7 @@ def load() -> C.CDLL:
     lib.effm_hpsi_multi.argtypes = [vp, i32, vp, i32, vp, vp, vp, i32, i32]
     lib.effm_dhdk_multi.argtypes = [vp, i32, vp, i32, vp, vp, vp, vp, vp, i32, i32]
     lib.effm_hpsi_dhdk_multi.argtypes = [vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, i32, i32]
+    lib.effm_lincomb.argtypes = [vp, i64, C.c_double, vp, C.c_double, vp, C.c_double, vp, i32, i32]
     lib.effm_rayleigh.argtypes = [vp, i32, i32, vp, i64, vp, i64, i32, vp, i32]
     lib.effm_autotune.argtypes = [vp, i32, i32, i32, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     lib.effm_overlap.argtypes = [vp, i32, i32, i32, vp, i64, vp, i64, vp, i64, i32, i32]

@@ -1013,6 +1013,22 @@ int effm_overlap(effm_ctx* h, int m, int n, int k, const void* A, int64_t lda, c
     return kOk;
 }
 
+int effm_lincomb(effm_ctx* h, int64_t n, double a, void* z, double b, const void* y, double cc, const void* x,
+                 int prec, int flags) {
+    if (!h) return kErrInvalid;
+    Ctx& c = h->c;
+    (void)hipSetDevice(c.device);
+    if (n < 0) return fail(c, kErrInvalid, "negative length");
+    if (n == 0) return kOk;
+    if (!z || !y || !x) return fail(c, kErrInvalid, "lincomb operands must not be NULL");
+    if (prec != EFFM_PREC_FP64 && prec != EFFM_PREC_MIXED) return fail(c, kErrInvalid, "unknown precision");
+    if (flags & (EFFM_F_HOST | EFFM_F_GRAPH)) return fail(c, kErrInvalid, "effm_lincomb takes device pointers only");
+    const hipError_t e = prec == EFFM_PREC_FP64 ? launch_lincomb<double>(c, n, a, z, b, y, cc, x)
+                                                : launch_lincomb<float>(c, n, a, z, b, y, cc, x);
+    if (e != hipSuccess) return hip_fail(c, e, "lincomb");
+    return kOk;
+}
+
 int effm_rayleigh(effm_ctx* h, int npw, int nb, const void* psi, int64_t ld, const void* hpsi,
                   int64_t ld_h, int prec, double* E, int flags) {
     if (!h) return kErrInvalid;

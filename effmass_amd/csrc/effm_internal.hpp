@@ -253,6 +253,10 @@ constexpr int kPencilBandChoices[] = {4, 8, 16};
 constexpr int kGemmWgChoices[] = {512, 768, 1024, 1536, 2048};
 
 template <typename T>
+hipError_t launch_lincomb(Ctx& c, int64_t n, double a, void* z, double b, const void* y, double cc,
+                          const void* x);
+
+template <typename T>
 hipError_t launch_rayleigh(hipStream_t s, int npw, int nb, const void* psi, int64_t ld,
                            const void* hpsi, int64_t ldh, double* E);
 

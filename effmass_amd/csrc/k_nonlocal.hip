@@ -931,4 +931,31 @@ template hipError_t launch_rayleigh<double>(hipStream_t, int, int, const void*, 
 template hipError_t launch_rayleigh<float>(hipStream_t, int, int, const void*, int64_t,
                                            const void*, int64_t, double*);
 
+// z <- a z + b y + c x elementwise over n complex values, in FP64 for either
+// storage precision: one pass where three scale / add passes would read z
+// three times (the eigensolver's Chebyshev three-term recurrence).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_lincomb(int64_t n, double a, cx_t<T>* __restrict__ z, double b, const cx_t<T>* __restrict__ y, double c,
+          const cx_t<T>* __restrict__ x) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const cx_t<T> zv = z[i], yv = y[i], xv = x[i];
+        z[i] = cx_t<T>{(T)(a * (double)zv.x + b * (double)yv.x + c * (double)xv.x),
+                       (T)(a * (double)zv.y + b * (double)yv.y + c * (double)xv.y)};
+    }
+}
+
+template <typename T>
+hipError_t launch_lincomb(Ctx& c, int64_t n, double a, void* z, double b, const void* y, double cc,
+                          const void* x) {
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, (int64_t)c.ncu * 16);
+    hipLaunchKernelGGL((k_lincomb<T>), dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0, c.stream, n, a,
+                       static_cast<cx_t<T>*>(z), b, static_cast<const cx_t<T>*>(y), cc,
+                       static_cast<const cx_t<T>*>(x));
+    return hipGetLastError();
+}
+template hipError_t launch_lincomb<double>(Ctx&, int64_t, double, void*, double, const void*, double, const void*);
+template hipError_t launch_lincomb<float>(Ctx&, int64_t, double, void*, double, const void*, double, const void*);
+
 }  // namespace effm

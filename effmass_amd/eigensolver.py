@@ -209,7 +209,8 @@ def eigh_host(A: torch.Tensor):
 
 
 def chfsi(apply, X0: torch.Tensor, kin: torch.Tensor, nconv: int | None = None, tol: float = 1e-8,
-          maxiter: int = 60, work_dtype=None, gram=None, degree: int = 12, eigh=None) -> EigResult:
+          maxiter: int = 60, work_dtype=None, gram=None, degree: int = 12, eigh=None,
+          lincomb=None) -> EigResult:
     """Chebyshev-filtered subspace iteration (Zhou, Saad, Tiago and
     Chelikowsky, J. Comput. Phys. 219 (2006) 172): each outer step filters
     the block with a degree-`degree` Chebyshev polynomial that damps
@@ -223,7 +224,9 @@ def chfsi(apply, X0: torch.Tensor, kin: torch.Tensor, nconv: int | None = None, 
     mixed path: it only has to damp the unwanted part of the spectrum); the
     orthonormalisation overlaps, the Rayleigh-Ritz matrices and the Ritz
     vectors are FP64.  eigh(A) solves the small Rayleigh-Ritz problems
-    (default torch.linalg.eigh; eigh_host runs them on the host)."""
+    (default torch.linalg.eigh; eigh_host runs them on the host).
+    lincomb(z, a, y, b, x, c) does z <- a z + b y + c x in place (default:
+    three torch ops; Hamiltonian.lincomb does it in one device pass)."""
     dev = X0.device
     cdt = torch.complex128
     wdt = work_dtype or X0.dtype
@@ -231,6 +234,7 @@ def chfsi(apply, X0: torch.Tensor, kin: torch.Tensor, nconv: int | None = None, 
     nconv = nb if nconv is None else nconv
     gr = gram or _gram
     eig = eigh or (lambda A: torch.linalg.eigh(0.5 * (A + A.conj().T)))
+    lc = lincomb or (lambda z, a, y, b, x, c: z.mul_(a).add_(y, alpha=b).add_(x, alpha=c))
 
     def Hw(V):                               # work precision in and out
         HV = torch.empty_like(V)
@@ -264,7 +268,7 @@ def chfsi(apply, X0: torch.Tensor, kin: torch.Tensor, nconv: int | None = None, 
             sig_new = 1.0 / (tau - sigma)
             # Y_next = (H - mid) Y 2 sig_new / half - sigma sig_new Y_prev, in place
             Yn = Hw(Y)
-            Yn.mul_(2.0 * sig_new / half).add_(Y, alpha=-mid * 2.0 * sig_new / half).add_(Xp, alpha=-sigma * sig_new)
+            lc(Yn, 2.0 * sig_new / half, Y, -mid * 2.0 * sig_new / half, Xp, -sigma * sig_new)
             Xp, Y, sigma = Y, Yn, sig_new
         Y = Y.to(cdt)
         try:

@@ -251,6 +251,21 @@ class Hamiltonian:
                    self._ctx, "effm_hpsi_dhdk_multi")
         return hpsis, outs
 
+    def lincomb(self, z, a, y, b, x, c):
+        """z <- a z + b y + c x in place for same-shaped contiguous device
+        tensors (complex128 or complex64), one pass on the device."""
+        import torch
+        if not (z.dtype == y.dtype == x.dtype) or z.dtype not in (torch.complex128, torch.complex64):
+            raise TypeError("lincomb takes three complex128 or three complex64 tensors")
+        if not (z.shape == y.shape == x.shape) or not (z.is_contiguous() and y.is_contiguous()
+                                                        and x.is_contiguous()):
+            raise ValueError("lincomb takes same-shaped contiguous tensors")
+        p = _lib.EFFM_PREC_FP64 if z.dtype == torch.complex128 else _lib.EFFM_PREC_MIXED
+        self._sync_stream()
+        _lib.check(self._lib.effm_lincomb(self._ctx, z.numel(), float(a), z.data_ptr(), float(b), y.data_ptr(),
+                                          float(c), x.data_ptr(), p, 0), self._ctx, "effm_lincomb")
+        return z
+
     def overlap(self, A, B):
         """C = conj(A) @ B.T for device tensors A [m, k], B [n, k] (complex128
         -> FP64 path, complex64 -> mixed), split-K sums added in FP64; C has

@@ -178,7 +178,7 @@ def run_query(q: MassQuery, prec="mixed", device=0, wl=None, budget=None) -> Mas
             # on the serving query mix, same eigenvalues to 1e-7 Ha)
             res = eigensolver.chfsi(lambda V, HV: op(V, HV, prec=prec, overwrite=True), X0, kin,
                                     nconv=min(nb, band + 2), tol=tol, maxiter=200, work_dtype=wdt,
-                                    gram=H.overlap, degree=20, eigh=eigensolver.eigh_host)
+                                    gram=H.overlap, degree=20, eigh=eigensolver.eigh_host, lincomb=H.lincomb)
             m = effmass.effective_mass(H, ik, kp, wl.cell.A, res.X.contiguous(), res.E, [band], prec=prec)
             inv.append(m.inv_mass[0])
             en.append(float(res.E[band]))

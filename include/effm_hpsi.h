@@ -162,6 +162,13 @@ int effm_hpsi_dhdk_multi(effm_ctx* ctx, int nk, const int* ikpts, int nb,
                          void* const* out_y, void* const* out_z, int prec,
                          int flags);
 
+/* z <- a z + b y + c x over n complex elements (device pointers, element
+ * type from prec, arithmetic in FP64) on the context's stream: the
+ * three-term recurrence of a Chebyshev filter in one pass.  No reference
+ * counterpart (the reference has no eigensolver). */
+int effm_lincomb(effm_ctx* ctx, int64_t n, double a, void* z, double b, const void* y,
+                 double c, const void* x, int prec, int flags);
+
 /* FP64 Rayleigh quotients E_b = Re<psi_b|hpsi_b> / <psi_b|psi_b> over npw
  * elements (test_hpsi_eigen.cpp:49-56 computes this on the host); E is a
  * device pointer unless EFFM_F_HOST.  prec selects the element type. */
