@@ -12,6 +12,19 @@
 
 namespace effm {
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on
+// XCD i % 8), and each XCD has its own L2.  xcd_order(i, G) renumbers a 1D
+// grid of G workgroups so that consecutive logical ids run on one XCD:
+// workgroups that read the same lines (GEMM tiles on the same operand rows,
+// neighbouring z-chunks of the pencil pass) are given consecutive ids and so
+// share one L2.  A bijection for any G (the remainder keeps its ids).
+constexpr int kXcds = 8;
+__device__ inline int xcd_order(int i, int G) {
+    const int per = G / kXcds;
+    if (i >= per * kXcds) return i;
+    return (i % kXcds) * per + i / kXcds;
+}
+
 // Status codes (mirrors include/effm_hpsi.h).
 enum : int {
     kOk = 0,

@@ -477,13 +477,21 @@ k_xpencil_vloc_ct(const LocalBatch<T> bt, const T* __restrict__ vloc, int n2, in
     static_assert(P1 >= P2, "roles = P1");
     __shared__ C ex[N1 * TZ];
     __shared__ C tw[N1];
+    // logical workgroup (z-chunk fastest, then row j, band group, segment),
+    // in XCD order: the z-chunks of one row share cache lines at their
+    // edges, and on one XCD they share its L2 (measured on the 64-atom step:
+    // FP64 fetch 1.98 -> 1.48 GB per launch against 1.36 GB read; time
+    // unchanged, the pass runs at the device's copy rate)
+    const int G = (int)(gridDim.x * gridDim.y);
+    const int L = xcd_order((int)(blockIdx.x + blockIdx.y * gridDim.x), G);
+    const int bx = L % (int)gridDim.x, by = L / (int)gridDim.x;
     const int nbg = (bt.nb + NBW - 1) / NBW;
-    const int sidx = blockIdx.y / nbg;
-    const int bgrp = blockIdx.y - sidx * nbg;
+    const int sidx = by / nbg;
+    const int bgrp = by - sidx * nbg;
     const LocalSeg<T>& sg = bt.s[sidx];
     const int nzc = (n3 + TZ - 1) / TZ;
-    const int j = blockIdx.x / nzc;
-    const int z0 = (blockIdx.x - j * nzc) * TZ;
+    const int j = bx / nzc;
+    const int z0 = (bx - j * nzc) * TZ;
     const int z = threadIdx.x % TZ;
     const int role = threadIdx.x / TZ;
     const bool zv = z0 + z < n3;

@@ -231,17 +231,6 @@ hipError_t launch_build_proj(Ctx& c, KptDev& k, bool grad) {
 template hipError_t launch_build_proj<double>(Ctx&, KptDev&, bool);
 template hipError_t launch_build_proj<float>(Ctx&, KptDev&, bool);
 
-// Workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on
-// XCD i % 8), and each XCD has its own L2.  xcd_order(i, G) renumbers a 1D
-// grid so that consecutive logical ids run on one XCD; tiles that read the
-// same operand rows are given consecutive ids and so share one L2.
-constexpr int kXcds = 8;
-__device__ inline int xcd_order(int i, int G) {
-    const int per = G / kXcds;
-    if (i >= per * kXcds) return i;
-    return (i % kXcds) * per + i / kXcds;
-}
-
 // ---------------------------------------------------------------------------
 //  GEMM 1: part[split][row0 + m][n] = sum_{k in split} conj(A[m][k]) B[n][k]
 //  A: [M][lda] (projectors, G contiguous), B: [N][ldb] (bands, G contiguous).
