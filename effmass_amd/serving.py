@@ -230,14 +230,15 @@ class MassServer:
     worker processes (at most 15, next to the caller's own process).
     A query with several k-points is admitted as one part per k-point (the
     k-points are independent eigenproblems on the same cell), so idle slots
-    shorten its latency; the parts' results are merged in k order.
+    shorten its latency; the parts' results are merged in k order.  Each
+    slot's host-side LAPACK and torch work uses host_threads threads.
     Timestamps are time.perf_counter() (CLOCK_MONOTONIC, the same clock in
     every process)."""
 
     MAX_SLOTS = 15
 
     def __init__(self, device=0, hbm_budget=None, slots=4, prec="mixed", start_timeout=600.0,
-                 split_kpoints=True):
+                 split_kpoints=True, host_threads=2):
         if not 1 <= slots <= self.MAX_SLOTS:
             raise ValueError(f"slots must be 1..{self.MAX_SLOTS}")
         self.device = device
@@ -248,7 +249,8 @@ class MassServer:
         ctx = mp.get_context("spawn")
         self._tasks = [ctx.Queue() for _ in range(slots)]
         self._results = ctx.Queue()
-        self._procs = [ctx.Process(target=_worker_main, args=(i, device, prec, self._tasks[i], self._results),
+        self._procs = [ctx.Process(target=_worker_main,
+                                   args=(i, device, prec, self._tasks[i], self._results, host_threads),
                                    daemon=True, name=f"mass-slot-{i}") for i in range(slots)]
         for p in self._procs:
             p.start()
@@ -415,11 +417,12 @@ class MassServer:
 
 
 def serve(queries, device=0, slots=4, prec="mixed", hbm_budget=None, realtime=True, timeout=None, progress=None,
-          split_kpoints=True):
+          split_kpoints=True, host_threads=2):
     """Replay `queries` at their arrival times (realtime) or all at once;
     return (answers by qid, stats).  progress(done, total) is called about
     once a second while waiting."""
-    srv = MassServer(device=device, hbm_budget=hbm_budget, slots=slots, prec=prec, split_kpoints=split_kpoints)
+    srv = MassServer(device=device, hbm_budget=hbm_budget, slots=slots, prec=prec, split_kpoints=split_kpoints,
+                     host_threads=host_threads)
     try:
         t0 = time.perf_counter()
         for q in sorted(queries, key=lambda x: x.arrival):
