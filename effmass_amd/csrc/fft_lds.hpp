@@ -217,6 +217,46 @@ __device__ __forceinline__ void dft_split(cx_t<T>* y) {
     }
 }
 
+// Inverse of a modulo m (a, m coprime), at compile time.
+constexpr int inv_mod(int a, int m) {
+    for (int x = 1; x < m; ++x)
+        if ((a * x) % m == 1) return x;
+    return 1;
+}
+constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+
+// Composite P = P1 * P2 with gcd(P1, P2) = 1 by the prime-factor (Good-
+// Thomas) mapping, which needs no twiddle factors: input u = (P2 u1 + P1 u2)
+// mod P, output q = (P2 a q1 + P1 b q2) mod P with a = P2^-1 mod P1,
+// b = P1^-1 mod P2 (q = q1 mod P1 = q2 mod P2), since then
+// u q = P2 u1 q1 + P1 u2 q2 (mod P) and the DFT separates into DFT_P1 over
+// u1 and DFT_P2 over u2.
+template <int P1, int P2, int S, typename T>
+__device__ __forceinline__ void dft_pfa(cx_t<T>* y) {
+    using C = cx_t<T>;
+    constexpr int P = P1 * P2;
+    constexpr int A = inv_mod(P2 % P1, P1), B = inv_mod(P1 % P2, P2);
+    C z[P];
+#pragma unroll
+    for (int u2 = 0; u2 < P2; ++u2) {
+        C a[P1];
+#pragma unroll
+        for (int u1 = 0; u1 < P1; ++u1) a[u1] = y[(P2 * u1 + P1 * u2) % P];
+        dft<P1, S, T>(a);
+#pragma unroll
+        for (int q1 = 0; q1 < P1; ++q1) z[u2 * P1 + q1] = a[q1];
+    }
+#pragma unroll
+    for (int q1 = 0; q1 < P1; ++q1) {
+        C b[P2];
+#pragma unroll
+        for (int u2 = 0; u2 < P2; ++u2) b[u2] = z[u2 * P1 + q1];
+        dft<P2, S, T>(b);
+#pragma unroll
+        for (int q2 = 0; q2 < P2; ++q2) y[(P2 * A * q1 + P1 * B * q2) % P] = b[q2];
+    }
+}
+
 template <int P, int S, typename T>
 __device__ __forceinline__ void dft(cx_t<T>* y) {
     if constexpr (P == 1) {
@@ -224,12 +264,12 @@ __device__ __forceinline__ void dft(cx_t<T>* y) {
     else if constexpr (P == 3) dft3<S, T>(y);
     else if constexpr (P == 4) dft4<S, T>(y);
     else if constexpr (P == 5) dft5<S, T>(y);
-    else if constexpr (P == 6) dft_split<2, 3, S, T>(y);
+    else if constexpr (P == 6) dft_pfa<2, 3, S, T>(y);
     else if constexpr (P == 8) dft_split<2, 4, S, T>(y);
     else if constexpr (P == 9) dft_split<3, 3, S, T>(y);
-    else if constexpr (P == 10) dft_split<2, 5, S, T>(y);
-    else if constexpr (P == 12) dft_split<3, 4, S, T>(y);
-    else if constexpr (P == 15) dft_split<3, 5, S, T>(y);
+    else if constexpr (P == 10) dft_pfa<2, 5, S, T>(y);
+    else if constexpr (P == 12) dft_pfa<3, 4, S, T>(y);
+    else if constexpr (P == 15) dft_pfa<3, 5, S, T>(y);
     else if constexpr (P == 16) dft_split<4, 4, S, T>(y);
     else if constexpr (P == 7) dft7<S, T>(y);
 }
