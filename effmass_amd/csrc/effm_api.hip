@@ -590,6 +590,19 @@ int effm_cell_upload(effm_ctx* h, const effm_cell* cell) {
     if (!(c.omega > 0)) return fail(c, kErrInvalid, "lattice is singular");
     c.ax1 = a1; c.ax2 = a2; c.ax3 = a3;
     c.hperm1 = fft_perm(a1); c.hperm2 = fft_perm(a2); c.hperm3 = fft_perm(a3);
+    int pp1 = 0, pp2 = 0;
+    if (plane_ct_pfa(c, &pp1, &pp2)) {
+        // prime-factor order of the plane kernels: natural j = (P2 u + P1 b)
+        // mod n sits at b P1 + u, i.e. u = (j mod P1) P2^-1 mod P1 and
+        // b = (j mod P2) P1^-1 mod P2
+        int inv1 = 1, inv2 = 1;
+        while ((pp2 * inv1) % pp1 != 1 % pp1) ++inv1;
+        while ((pp1 * inv2) % pp2 != 1 % pp2) ++inv2;
+        for (int j = 0; j < c.n2; ++j) {
+            const int u = ((j % pp1) * inv1) % pp1, b = ((j % pp2) * inv2) % pp2;
+            c.hperm2[j] = c.hperm3[j] = b * pp1 + u;
+        }
+    }
     int rc;
     size_t N = (size_t)c.n1 * c.n2 * c.n3;
     std::vector<double> v64(cell->vloc, cell->vloc + N);

@@ -235,6 +235,10 @@ hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, 
 template <typename T>
 size_t local_work_bytes(const Ctx& c, const ApplySeg* segs, int nseg, int nb);
 
+// True if the compile-time square-plane kernels serve this cell with
+// coprime radices p1 x p2 = n2 (then in prime-factor order, see fft_ct.hpp).
+bool plane_ct_pfa(const Ctx& c, int* p1, int* p2);
+
 // ---- launchers (k_nonlocal.hip) -----------------------------------------
 template <typename T>
 hipError_t launch_build_proj(Ctx& c, KptDev& k, bool grad);

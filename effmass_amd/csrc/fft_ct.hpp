@@ -29,6 +29,23 @@ EFFM_PLAN2(48, 8, 6)
 EFFM_PLAN2(90, 10, 9)
 #undef EFFM_PLAN2
 
+// Prime-factor axes: with gcd(P1, P2) = 1 the two stages need no twiddle
+// factors if the inputs sit in the Good-Thomas order and the outputs are
+// read in the Chinese-remainder order (see dft_pfa).  In place, input
+// natural index (P2 u + P1 b) mod N sits at b P1 + u (the host builds the
+// plane scatter / gather positions that way), and after both DIT stages
+// position k1 + P1 q holds natural index nat(k1, q) = (P2 a k1 + P1 c q)
+// mod N (a = P2^-1 mod P1, c = P1^-1 mod P2), i.e. natural index x sits
+// at pos(x) = x mod P1 + P1 (x mod P2).  The DIF stages run the same maps
+// backwards.
+template <int N> struct Pfa {
+    static constexpr int P1 = Plan2<N>::P1, P2 = Plan2<N>::P2;
+    static constexpr bool on = Plan2<N>::ok && gcd_c(P1, P2) == 1;
+    static constexpr int A = inv_mod(P2 % P1, P1), C = inv_mod(P1 % P2, P2);
+    static __device__ __forceinline__ int nat(int k1, int q) { return (P2 * A * k1 + P1 * C * q) % N; }
+    static __device__ __forceinline__ int pos(int x) { return x % P1 + P1 * (x % P2); }
+};
+
 // The axis lengths above (the 64-atom cell's 90, the 8-atom 48, and the
 // grids of 2-64 atom cells at 30-50 Ry); X-macro for dispatch.
 #define EFFM_CT_SIZES(X) X(15) X(16) X(18) X(20) X(24) X(25) X(36) X(40) X(45) X(48) X(90)
@@ -58,7 +75,7 @@ __device__ __forceinline__ void ct_stage(cx_t<T>* buf, const cx_t<T>* tw) {
             const int b = r / L;
             const int k1 = r - b * L;
             const int base = line * LS + (b * LT + k1) * ES;
-            butterfly<P, S, DIT, T>(buf, base, L * ES, k1, TSTEP, tw);
+            butterfly<P, S, DIT, T>(buf, base, L * ES, Pfa<N>::on ? 0 : k1, TSTEP, tw);
         }
     }
 }
@@ -99,7 +116,7 @@ __device__ __forceinline__ void ct_stage_rows(cx_t<T>* buf, const cx_t<T>* tw, c
             const int r = t - li * NBF;
             const int b = r / L;
             const int k1 = r - b * L;
-            butterfly<P, S, DIT, T>(buf, rows[li] * N + b * LT + k1, L, k1, TSTEP, tw);
+            butterfly<P, S, DIT, T>(buf, rows[li] * N + b * LT + k1, L, Pfa<N>::on ? 0 : k1, TSTEP, tw);
         }
     }
 }
@@ -159,16 +176,20 @@ __device__ __forceinline__ void ct_cols_last_to_global(const cx_t<T>* buf, const
         if (Cs::ITER * BLOCK == Cs::TOTAL || t < Cs::TOTAL) {
             const int r = t / WP, col = t - r * WP;
             const int c = min(col, N - 1);
+            const int cp = Pfa<N>::on ? Pfa<N>::pos(c) : c;
             cx_t<T> y[Cs::P2];
 #pragma unroll
-            for (int u = 0; u < Cs::P2; ++u) y[u] = buf[(r + u * Cs::P1) * N + c];
-            if (r != 0) {
+            for (int u = 0; u < Cs::P2; ++u) y[u] = buf[(r + u * Cs::P1) * N + cp];
+            if (!Pfa<N>::on && r != 0) {
 #pragma unroll
                 for (int u = 1; u < Cs::P2; ++u) y[u] = cmul(y[u], twid<S>(tw, u * r));
             }
             dft<Cs::P2, S, T>(y);
 #pragma unroll
-            for (int u = 0; u < Cs::P2; ++u) store_stream(y[u], &out[(int64_t)(r + u * Cs::P1) * WP + col]);
+            for (int u = 0; u < Cs::P2; ++u) {
+                const int row = Pfa<N>::on ? Pfa<N>::nat(r, u) : r + u * Cs::P1;
+                store_stream(y[u], &out[(int64_t)row * WP + col]);
+            }
         }
     }
 }
@@ -185,7 +206,10 @@ __device__ __forceinline__ void ct_cols_first_load(const cx_t<T>* in,
         const int t = min((int)threadIdx.x + it * BLOCK, Cs::TOTAL - 1);
         const int r = t / N, col = t - r * N;
 #pragma unroll
-        for (int u = U0; u < U1; ++u) y[it][u] = in[(int64_t)(r + u * Cs::P1) * WP + col];
+        for (int u = U0; u < U1; ++u) {
+            const int row = Pfa<N>::on ? Pfa<N>::nat(r, u) : r + u * Cs::P1;
+            y[it][u] = in[(int64_t)row * WP + col];
+        }
     }
 }
 
@@ -201,12 +225,13 @@ __device__ __forceinline__ void ct_cols_first_to_lds(cx_t<T>* buf, const cx_t<T>
         if (Cs::ITER * BLOCK == Cs::TOTAL || t < Cs::TOTAL) {
             const int r = t / N, col = t - r * N;
             dft<Cs::P2, S, T>(y[it]);
-            if (r != 0) {
+            if (!Pfa<N>::on && r != 0) {
 #pragma unroll
                 for (int u = 1; u < Cs::P2; ++u) y[it][u] = cmul(y[it][u], twid<S>(tw, u * r));
             }
+            const int cp = Pfa<N>::on ? Pfa<N>::pos(col) : col;
 #pragma unroll
-            for (int u = 0; u < Cs::P2; ++u) buf[(r + u * Cs::P1) * N + col] = y[it][u];
+            for (int u = 0; u < Cs::P2; ++u) buf[(r + u * Cs::P1) * N + cp] = y[it][u];
         }
     }
 }

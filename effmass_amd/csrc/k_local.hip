@@ -657,6 +657,20 @@ static bool plan_is(const FftAxis& a, int p1, int p2) {
     return a.ns == 2 && a.p[0] == p1 && a.p[1] == p2;
 }
 
+// The square-plane kernels with compile-time plans serve this cell, with
+// coprime radices: their scatter / gather positions are then the
+// prime-factor order of Pfa<N> (p1, p2 = the radices).
+bool plane_ct_pfa(const Ctx& c, int* p1, int* p2) {
+    const int a = Plan2N(c.n2, 1), b = Plan2N(c.n2, 2);
+    if (c.n2 != c.n3 || a <= 0 || !plan_is(c.ax2, a, b) || !plan_is(c.ax3, a, b)) return false;
+    int x = a, y = b;
+    while (y) { const int t = x % y; x = y; y = t; }
+    if (x != 1) return false;
+    *p1 = a;
+    *p2 = b;
+    return true;
+}
+
 // Persistent grid: as many workgroups as the occupancy calculator says are
 // resident at once (LDS, VGPRs, wave slots), capped by the items.  The
 // dynamic-LDS attribute and the occupancy are set up once per kernel and
