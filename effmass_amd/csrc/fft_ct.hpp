@@ -165,6 +165,20 @@ __device__ __forceinline__ void store_stream(C v, C* p) {
     __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
 }
 
+// Streaming (nontemporal) load of one complex value: a plane is read once,
+// so it should not push the arrays every plane reuses (sphere indices,
+// kinetic energies, psi) out of L2.  (Measured on the 64-atom R2G pass:
+// 249 -> 213 us mixed, 412 -> 403 us FP64, fetch 1003 -> 969 MB mixed.  In
+// the x-pencil pass, whose neighbouring z-chunks share lines, they were
+// slower: FP64 570 -> 726 us, fetch 1.55 -> 1.93 GB.)
+template <typename C>
+__device__ __forceinline__ C load_stream(const C* p) {
+    using E = decltype(p->x);
+    typedef E v2 __attribute__((ext_vector_type(2)));
+    const v2 w = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return C{w.x, w.y};
+}
+
 // G2R: last DIT stage of the columns, from LDS to out[row * WP + col].
 // The pad columns N <= col < WP repeat column N - 1 (whole rows are written).
 template <int N, int WP, int S, int BLOCK, typename T>
@@ -208,7 +222,7 @@ __device__ __forceinline__ void ct_cols_first_load(const cx_t<T>* in,
 #pragma unroll
         for (int u = U0; u < U1; ++u) {
             const int row = Pfa<N>::on ? Pfa<N>::nat(r, u) : r + u * Cs::P1;
-            y[it][u] = in[(int64_t)row * WP + col];
+            y[it][u] = load_stream(&in[(int64_t)row * WP + col]);
         }
     }
 }
