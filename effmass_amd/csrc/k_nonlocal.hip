@@ -242,11 +242,15 @@ template hipError_t launch_build_proj<float>(Ctx&, KptDev&, bool);
 // ---------------------------------------------------------------------------
 template <typename T>
 struct ProjSeg {
-    const cx_t<T>* A;      // [M][lda]
+    const cx_t<T>* A;      // rows [0, M1): [M1][lda]
     const cx_t<T>* B;      // [N][ldb]
     cx_t<T>* part;         // [nsplit][Mtot][N]
     int64_t lda, ldb;
     int K, kchunk;
+    // rows [M1, M) from A2 ([M - M1][lda], the gradient's dP after P); by
+    // default every row comes from A
+    const cx_t<T>* A2 = nullptr;
+    int M1 = 1 << 30;
 };
 
 template <typename T>
@@ -311,7 +315,8 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
     for (int q = 0; q < EPT; ++q) {
         const int e = threadIdx.x + 256 * q;
         const int row = e / BK, col = e - row * BK;
-        ga[q] = A + (int64_t)min(tm0 + row, M - 1) * lda;
+        const int gr = min(tm0 + row, M - 1);
+        ga[q] = (sg.A2 != nullptr && gr >= sg.M1) ? sg.A2 + (int64_t)(gr - sg.M1) * lda : A + (int64_t)gr * lda;
         gb[q] = B + (int64_t)min(tn0 + row, N - 1) * ldb;
         lofs[q] = row * LDK + col;
         kofs[q] = col;
@@ -656,10 +661,14 @@ static int min_npw(const ApplySeg* segs, int nseg) {
 }
 
 // <A_s|psi_s> for every segment: rows [row0, row0 + M) of each segment's
-// partial buffer part_s = partial + s * nsplit * Mtot * nb.
+// partial buffer part_s = partial + s * nsplit * Mtot * nb.  With A2, rows
+// [M1, M) of the product come from A2 (the projector k-gradients after the
+// projectors: one launch for all 4 nch rows, so the split fills the device
+// for the whole projection rather than for each part).
 template <typename T>
 static hipError_t gemm_proj(Ctx& c, const ApplySeg* segs, int nseg, const cx_t<T>* const* A, int M,
-                            int nb, cx_t<T>* partial, int Mtot, int row0, int nsplit) {
+                            int nb, cx_t<T>* partial, int Mtot, int row0, int nsplit,
+                            const cx_t<T>* const* A2 = nullptr, int M1 = 0) {
     ProjBatch<T> bt{};
     bt.nseg = nseg;
     for (int i = 0; i < nseg; ++i) {
@@ -667,7 +676,8 @@ static hipError_t gemm_proj(Ctx& c, const ApplySeg* segs, int nseg, const cx_t<T
         int kchunk = (npw + nsplit - 1) / nsplit;
         kchunk = (kchunk + 15) / 16 * 16;
         bt.s[i] = ProjSeg<T>{A[i], static_cast<const cx_t<T>*>(segs[i].psi),
-                             partial + (size_t)i * nsplit * Mtot * nb, (int64_t)npw, segs[i].ld, npw, kchunk};
+                             partial + (size_t)i * nsplit * Mtot * nb, (int64_t)npw, segs[i].ld, npw, kchunk,
+                             A2 ? A2[i] : nullptr, A2 ? M1 : M};
     }
     dim3 grid(((M + 63) / 64) * ((nb + 63) / 64) * nsplit * nseg);
     hipLaunchKernelGGL((k_gemm_proj<T>), grid, dim3(256), 0, c.stream, bt, M, nb, nsplit, Mtot, row0);
@@ -779,10 +789,9 @@ hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accu
     if ((size_t)nsplit * Mtot * nb * sizeof(C) * nseg > partial_bytes) return hipErrorInvalidValue;
     C* part = static_cast<C*>(partial);
     C* cb = static_cast<C*>(cbuf);
-    // <p|psi> and <dp_a|psi> for a = x, y, z in one partial buffer
+    // <p|psi> and <dp_a|psi> for a = x, y, z in one launch into one partial buffer
     timer_begin(c);
-    e = gemm_proj<T>(c, segs, nseg, P, nch, nb, part, Mtot, 0, nsplit);
-    if (e == hipSuccess) e = gemm_proj<T>(c, segs, nseg, dP, 3 * nch, nb, part, Mtot, nch, nsplit);
+    e = gemm_proj<T>(c, segs, nseg, P, Mtot, nb, part, Mtot, 0, nsplit, dP, nch);
     timer_end(c, "proj_gemm");
     if (e != hipSuccess) return e;
     timer_begin(c);
@@ -845,7 +854,7 @@ hipError_t launch_overlap(Ctx& c, int m, int n, int k, const void* A, int64_t ld
     ProjBatch<T> bt{};
     bt.nseg = 1;
     bt.s[0] = ProjSeg<T>{static_cast<const Cx*>(A), static_cast<const Cx*>(B), static_cast<Cx*>(partial), lda, ldb,
-                         k, kchunk};
+                         k, kchunk, nullptr, m};
     timer_begin(c);
     dim3 grid(((m + 63) / 64) * ((n + 63) / 64) * nsplit);
     hipLaunchKernelGGL((k_gemm_proj<T>), grid, dim3(256), 0, c.stream, bt, m, n, nsplit, m, 0);
