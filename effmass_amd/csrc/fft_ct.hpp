@@ -143,7 +143,9 @@ __device__ __forceinline__ void ct_fft_rows(cx_t<T>* buf, const cx_t<T>* tw, con
 // DIF stage taking its inputs from one.  With N = P1 * P2 that stage has P1
 // butterflies per column; work item t takes column t % W (consecutive lanes
 // on consecutive columns, so the global accesses of one instruction are row
-// segments) and butterfly r = t / W, whose elements are rows r + u * P1.
+// segments) and butterfly r = t / W, whose elements are rows r + u * P1 in
+// LDS (in global memory rows Pfa<N>::nat(r, u) on prime-factor axes, whose
+// LDS columns are then permuted too: natural column x at Pfa<N>::pos(x)).
 // Each skips an LDS round trip and a barrier against ct_fft + a copy.
 template <int N, int W, int BLOCK>
 struct ColStage {
