@@ -249,7 +249,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 FFT/GEMM, fp64 D contraction + Rayleigh",
+        "dtype": "f32",
+        "precision": "f32 FFT / GEMM (complex64 psi), f64 D contraction, kinetic term, Rayleigh quotients",
         "data": "synthetic (seeded SplitMix64, effmass_amd/synth.py workload si64_cubic)",
         "config": {"workload": "Si64 2x2x2 diamond supercell, a=20.52 bohr, ecut 40 Ry, k=(0.1,0,0), "
                                "npw 36945, FFT 90^3, 256 bands, 512 projectors; step = H|psi> on all bands",
