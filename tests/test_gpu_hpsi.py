@@ -299,3 +299,32 @@ def test_fft_radix_plans(grid):
     e = band_rel(got, ref)
     log(f"grid {grid} rel err {e:.2e}")
     assert e <= 1e-12
+
+
+@pytest.mark.parametrize("grid,ecut,nb", [((24, 24, 24), 2.0, 3), ((24, 24, 24), 12.0, 67),
+                                          ((36, 36, 36), 30.0, 300)])
+def test_odd_sizes(grid, ecut, nb):
+    """A sphere of a few plane waves, band counts that are not multiples of
+    the GEMM tiles (64) or of the pencil band groups, and more bands than
+    four column tiles: FP64 against the oracle at 1e-12; the mixed path's
+    Rayleigh quotients within 1e-6 Ry of the FP64 path's."""
+    torch = _torch()
+    wl = synth.custom(*synth.fcc_primitive(10.26), grid, ecut, [np.array([0.1, -0.2, 0.05])], nb,
+                      seed=int(ecut * 10) + nb)
+    kp = wl.kpts[0]
+    log(f"grid {grid} ecut {ecut} npw {kp.npw} nb {nb}")
+    got = run_h(wl)
+    ref = oracle.hpsi(wl.cell, kp, wl.psi[0])
+    assert band_rel(got, ref) <= 1e-12
+    H = Hamiltonian(wl.cell)
+    H.load_kpoint(0, kp)
+    psi = torch.from_numpy(wl.psi[0]).cuda()
+    h = torch.zeros_like(psi)
+    H.at_k(0)(psi, h)
+    p32 = psi.to(torch.complex64)
+    h32 = torch.zeros_like(p32)
+    H.at_k(0)(p32, h32, prec="mixed")
+    de = np.abs(H.rayleigh(p32, h32, prec="mixed").cpu().numpy() - H.rayleigh(psi, h).cpu().numpy()).max()
+    log(f"  mixed |dE| max {de:.2e} Ha")
+    assert de <= RY_TOL_HA
+    H.close()
