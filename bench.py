@@ -64,7 +64,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-bands", type=int, default=32, help="bands in the 1-core CPU sample")
+    ap.add_argument("--cpu-bands", type=int, default=96,
+                    help="bands in the 1-core CPU sample (about 12 s of CPU work at the 64-atom size)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--serve-queries", type=int, default=1000,
