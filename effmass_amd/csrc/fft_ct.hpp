@@ -181,10 +181,12 @@ __device__ __forceinline__ C load_stream(const C* p) {
     return C{w.x, w.y};
 }
 
-// G2R: last DIT stage of the columns, from LDS to out[row * WP + col].
-// The pad columns N <= col < WP repeat column N - 1 (whole rows are written).
+// G2R: last DIT stage of the columns, from LDS to out[row * rs + col]
+// (rs: the workspace's row stride, >= WP).  The pad columns N <= col < WP
+// repeat column N - 1 (whole rows are written).
 template <int N, int WP, int S, int BLOCK, typename T>
-__device__ __forceinline__ void ct_cols_last_to_global(const cx_t<T>* buf, const cx_t<T>* tw, cx_t<T>* out) {
+__device__ __forceinline__ void ct_cols_last_to_global(const cx_t<T>* buf, const cx_t<T>* tw, cx_t<T>* out,
+                                                       int64_t rs) {
     using Cs = ColStage<N, WP, BLOCK>;
 #pragma unroll
     for (int it = 0; it < Cs::ITER; ++it) {
@@ -204,18 +206,19 @@ __device__ __forceinline__ void ct_cols_last_to_global(const cx_t<T>* buf, const
 #pragma unroll
             for (int u = 0; u < Cs::P2; ++u) {
                 const int row = Pfa<N>::on ? Pfa<N>::nat(r, u) : r + u * Cs::P1;
-                store_stream(y[u], &out[(int64_t)row * WP + col]);
+                store_stream(y[u], &out[(int64_t)row * rs + col]);
             }
         }
     }
 }
 
 // R2G: the inputs u in [U0, U1) of this thread's first DIF column
-// butterflies, from in[row * WP + col], col < N.  Spare work items load a
+// butterflies, from in[row * rs + col], col < N.  Spare work items load a
 // valid duplicate.
 template <int N, int WP, int BLOCK, typename T, int U0 = 0, int U1 = Plan2<N>::P2>
 __device__ __forceinline__ void ct_cols_first_load(const cx_t<T>* in,
-                                                   cx_t<T> (&y)[ColStage<N, N, BLOCK>::ITER][Plan2<N>::P2]) {
+                                                   cx_t<T> (&y)[ColStage<N, N, BLOCK>::ITER][Plan2<N>::P2],
+                                                   int64_t rs) {
     using Cs = ColStage<N, N, BLOCK>;
 #pragma unroll
     for (int it = 0; it < Cs::ITER; ++it) {
@@ -224,7 +227,7 @@ __device__ __forceinline__ void ct_cols_first_load(const cx_t<T>* in,
 #pragma unroll
         for (int u = U0; u < U1; ++u) {
             const int row = Pfa<N>::on ? Pfa<N>::nat(r, u) : r + u * Cs::P1;
-            y[it][u] = load_stream(&in[(int64_t)row * WP + col]);
+            y[it][u] = load_stream(&in[(int64_t)row * rs + col]);
         }
     }
 }

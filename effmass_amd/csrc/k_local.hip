@@ -44,7 +44,14 @@ struct LocalSeg {
     const int* row_ptr;       // [nplanes + 1]
     const int* rows;          // rows of each plane that hold sphere points
     const double* kin;        // [npw]
-    cx_t<T>* work;            // [nb][nplanes][ws_plane]
+    cx_t<T>* work;            // [nb][nplanes][ws_plane] or [nb][n2][nplanes][pitch]
+    // element (band b, plane p, row j, z) of the compile-time kernels sits
+    // at work[b nplanes ws_plane + p wsp + j wsr + z]: plane-major
+    // (wsp = ws_plane, wsr = pitch) or row-major (wsp = pitch,
+    // wsr = nplanes pitch: the x-pencil pass of one row j reads one
+    // contiguous block of all planes); the runtime-plan kernels use
+    // plane-major
+    int64_t wsp, wsr;
     int nplanes;
     int item0;                // first (plane, band) item of the segment
 };
@@ -328,7 +335,7 @@ k_plane_g2r_ct(const LocalBatch<T> bt, int nitems, const cx_t<T>* __restrict__ t
         __syncthreads();
         if (nxt < nitems) fetch_values(q);
         ct_cols_last_to_global<N, WP, +1, BLOCK, T>(
-            plane, tw, cur.sg->work + ((int64_t)cur.b * cur.sg->nplanes + cur.p) * PS);
+            plane, tw, cur.sg->work + (int64_t)cur.b * cur.sg->nplanes * PS + cur.p * cur.sg->wsp, cur.sg->wsr);
         __syncthreads();
     }
 }
@@ -357,10 +364,10 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
     C y[Cs::ITER][Cs::P2];
     int nrows = 0, srow = 0;
     auto plane_of = [&](const PlaneItem<T>& q) {
-        return static_cast<const C*>(q.sg->work) + ((int64_t)q.b * q.sg->nplanes + q.p) * PS;
+        return static_cast<const C*>(q.sg->work) + (int64_t)q.b * q.sg->nplanes * PS + q.p * q.sg->wsp;
     };
     auto fetch = [&](const PlaneItem<T>& q) {
-        ct_cols_first_load<N, WP, BLOCK, T, 0, YP>(plane_of(q), y);
+        ct_cols_first_load<N, WP, BLOCK, T, 0, YP>(plane_of(q), y, q.sg->wsr);
         const int r0 = q.sg->row_ptr[q.p];
         nrows = q.sg->row_ptr[q.p + 1] - r0;
         if (threadIdx.x < nrows) srow = q.sg->rows[r0 + threadIdx.x];
@@ -370,7 +377,7 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
     for (; item < nitems; item += gridDim.x) {
         const PlaneItem<T> cur = plane_item(bt, item);
         const LocalSeg<T>& sg = *cur.sg;
-        if constexpr (YP < Cs::P2) ct_cols_first_load<N, WP, BLOCK, T, YP, Cs::P2>(plane_of(cur), y);
+        if constexpr (YP < Cs::P2) ct_cols_first_load<N, WP, BLOCK, T, YP, Cs::P2>(plane_of(cur), y, sg.wsr);
         ct_cols_first_to_lds<N, -1, BLOCK, T>(plane, tw, y);
         const int nr = nrows;
         if (threadIdx.x < nr) srows[threadIdx.x] = srow;
@@ -531,13 +538,13 @@ k_xpencil_vloc_ct(const LocalBatch<T> bt, const T* __restrict__ vloc, int n2, in
     // band's workspace and whether the plane is stored (active) at all
     const int64_t wps = ws_plane<T>(n2, n3);
     const int64_t band_bytes = (int64_t)sg.nplanes * wps * (int64_t)sizeof(C);
-    const unsigned colb = (unsigned)(((int64_t)j * ws_pitch<T>(n3) + z0 + z) * (int64_t)sizeof(C));
+    const unsigned colb = (unsigned)(((int64_t)j * sg.wsr + z0 + z) * (int64_t)sizeof(C));
     // (inactive: offset kNone).  HOIST keeps them in registers across the
     // bands; otherwise each band recomputes them from the plane map in LDS.
     constexpr bool HOIST = PencilShape<T, N1>::HOIST;
     constexpr unsigned kNone = ~0u;
     __shared__ int srcpl[HOIST ? 1 : N1];
-    const unsigned plane_bytes = (unsigned)(wps * (int64_t)sizeof(C));
+    const unsigned plane_bytes = (unsigned)(sg.wsp * (int64_t)sizeof(C));
     unsigned boff[P1];
     bool actr[P1];
     if constexpr (HOIST) {
@@ -769,12 +776,26 @@ size_t local_work_bytes(const Ctx& c, const ApplySeg* segs, int nseg, int nb) {
     return tot;
 }
 
+// Workspace of the compile-time kernels row-major ([band][row][plane][z])
+// rather than plane-major ([band][plane][row][z]).
+constexpr bool kRowMajorWork = true;
+
 template <typename T>
 hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, bool accumulate,
                         void* work) {
     using C = cx_t<T>;
     constexpr int TZ = 16;
     if (nseg < 1 || nseg > kMaxSeg || nb < 1) return hipErrorInvalidValue;
+    // compile-time plans for square planes / pencil lengths of EFFM_CT_SIZES
+    const bool plct = c.n2 == c.n3 && Plan2N(c.n2, 1) > 0 && plan_is(c.ax2, Plan2N(c.n2, 1), Plan2N(c.n2, 2)) &&
+                      plan_is(c.ax3, Plan2N(c.n3, 1), Plan2N(c.n3, 2));
+    // (the specialised pencil kernel addresses one band's workspace with
+    // 32-bit byte offsets)
+    const bool pect = Plan2N(c.n1, 1) > 0 && plan_is(c.ax1, Plan2N(c.n1, 1), Plan2N(c.n1, 2)) &&
+                      (int64_t)c.n1 * ws_plane<T>(c.n2, c.n3) * (int64_t)sizeof(C) < (int64_t{1} << 32);
+    // row-major workspace when the compile-time plane and pencil kernels
+    // both run (they alone read the strides)
+    const bool rowmaj = kRowMajorWork && plct && pect;
     LocalBatch<T> bt{};
     bt.nseg = nseg;
     bt.nb = nb;
@@ -797,6 +818,8 @@ hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, 
         s.kin = static_cast<const double*>(k.kin.ptr);
         s.work = wcur;
         s.nplanes = k.nplanes;
+        s.wsp = rowmaj ? (int64_t)ws_pitch<T>(c.n3) : ws_plane<T>(c.n2, c.n3);
+        s.wsr = rowmaj ? (int64_t)k.nplanes * ws_pitch<T>(c.n3) : (int64_t)ws_pitch<T>(c.n3);
         s.item0 = nitems;
         wcur += (int64_t)k.nplanes * ws_plane<T>(c.n2, c.n3) * nb;
         nitems += k.nplanes * nb;
@@ -812,13 +835,6 @@ hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, 
     const size_t lds_pencil = sizeof(C) * ((size_t)c.n1 * TZ + c.n1);
     hipError_t e;
 
-    // compile-time plans for square planes / pencil lengths of EFFM_CT_SIZES
-    const bool plct = c.n2 == c.n3 && Plan2N(c.n2, 1) > 0 && plan_is(c.ax2, Plan2N(c.n2, 1), Plan2N(c.n2, 2)) &&
-                      plan_is(c.ax3, Plan2N(c.n3, 1), Plan2N(c.n3, 2));
-    // (the specialised pencil kernel addresses one band's workspace with
-    // 32-bit byte offsets)
-    const bool pect = Plan2N(c.n1, 1) > 0 && plan_is(c.ax1, Plan2N(c.n1, 1), Plan2N(c.n1, 2)) &&
-                      (int64_t)c.n1 * ws_plane<T>(c.n2, c.n3) * (int64_t)sizeof(C) < (int64_t{1} << 32);
     if (!plct || !pect) {
         // generic kernels: dynamic LDS sized by this cell's grid
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_g2r<T>),
