@@ -306,8 +306,9 @@ def test_fft_radix_plans(grid):
 def test_odd_sizes(grid, ecut, nb):
     """A sphere of a few plane waves, band counts that are not multiples of
     the GEMM tiles (64) or of the pencil band groups, and more bands than
-    four column tiles: FP64 against the oracle at 1e-12; the mixed path's
-    Rayleigh quotients within 1e-6 Ry of the FP64 path's."""
+    four column tiles: FP64 H|psi> and dH/dk against the oracle at 1e-12;
+    the mixed path's Rayleigh quotients within 1e-6 Ry of the FP64 path's
+    and its dH/dk within 2e-5."""
     torch = _torch()
     wl = synth.custom(*synth.fcc_primitive(10.26), grid, ecut, [np.array([0.1, -0.2, 0.05])], nb,
                       seed=int(ecut * 10) + nb)
@@ -327,4 +328,15 @@ def test_odd_sizes(grid, ecut, nb):
     de = np.abs(H.rayleigh(p32, h32, prec="mixed").cpu().numpy() - H.rayleigh(psi, h).cpu().numpy()).max()
     log(f"  mixed |dE| max {de:.2e} Ha")
     assert de <= RY_TOL_HA
+    # dH/dk (one projection launch over the projector and k-gradient rows):
+    # FP64 at 1e-12, mixed at 2e-5 against the oracle
+    ref = oracle.dhdk(wl.cell, kp, wl.psi[0])
+    outs = [torch.zeros_like(psi) for _ in range(3)]
+    H.gradient().at_k(0)(psi, GradHPsi(*outs))
+    outs32 = [torch.zeros_like(p32) for _ in range(3)]
+    H.gradient().at_k(0)(p32, GradHPsi(*outs32), prec="mixed")
+    torch.cuda.synchronize()
+    for a in range(3):
+        assert band_rel(outs[a].cpu().numpy(), ref[a]) <= 1e-12
+        assert band_rel(outs32[a].cpu().numpy(), ref[a]) <= 2e-5
     H.close()
