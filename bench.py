@@ -326,6 +326,23 @@ def multik_leg(steps, warmup=1):
         t_eager = timed(lambda: step(False), steps)
         t_sep = timed(separate, steps)
         t_loop = timed(per_k, steps)
+        # parity of the timed path: a graph replay of the batch against
+        # single-k effm_hpsi + effm_dhdk calls on the first bands of two
+        # k-points (one in each launch group of 16)
+        step(True)
+        nchk = 4
+        worst = 0.0
+        for ik in (0, len(ks) - 1):
+            p = psis[ik][:nchk].contiguous()
+            sh = torch.zeros_like(p)
+            sg = [torch.zeros_like(p) for _ in range(3)]
+            H.at_k(ik)(p, sh, prec=prec, overwrite=True)
+            H.gradient().at_k(ik)(p, GradHPsi(*sg), prec=prec, overwrite=True)
+            for got, ref in [(hs[ik], sh), (gr.x[ik], sg[0]), (gr.y[ik], sg[1]), (gr.z[ik], sg[2])]:
+                d = (got[:nchk] - ref).abs().pow(2).sum(1).sqrt() / ref.abs().pow(2).sum(1).sqrt()
+                worst = max(worst, float(d.max()))
+        parity = {"max_rel_err_batch_graph_vs_single_k": worst, "kpoints_checked": [0, len(ks) - 1],
+                  "bands_checked": nchk, "outputs": "H|psi>, dH/dk_x,y,z|psi>"}
         # per-kernel split of one eager batched step (HIP events)
         H.reset_timing()
         H.set_timing(True)
@@ -340,7 +357,7 @@ def multik_leg(steps, warmup=1):
         applies = len(ks) * nb
         res[prec] = {"ms_per_step": t_graph * 1e3, "k_band_steps_per_s": applies / t_graph,
                      "eager_batched_ms": t_eager * 1e3, "separate_calls_graph_ms": t_sep * 1e3,
-                     "per_k_calls_ms": t_loop * 1e3, "kernel_ms": split}
+                     "per_k_calls_ms": t_loop * 1e3, "kernel_ms": split, "parity": parity}
         del psis, hs, gr
         torch.cuda.empty_cache()
     H.close()

@@ -1047,6 +1047,7 @@ int effm_rayleigh(effm_ctx* h, int npw, int nb, const void* psi, int64_t ld, con
     if (!h) return kErrInvalid;
     Ctx& c = h->c;
     (void)hipSetDevice(c.device);
+    if (prec != EFFM_PREC_FP64 && prec != EFFM_PREC_MIXED) return fail(c, kErrInvalid, "unknown precision");
     if (nb <= 0) return kOk;
     if (!psi || !hpsi || !E || npw <= 0 || ld < npw || ld_h < npw)
         return fail(c, kErrInvalid, "invalid Rayleigh-quotient arguments");

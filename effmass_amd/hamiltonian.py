@@ -32,8 +32,8 @@ def _is_torch(x) -> bool:
     return type(x).__module__.startswith("torch")
 
 
-def _as_2d_info(x, prec: int):
-    """(pointer, nb, ld, host?) of a band-major psi-like array."""
+def _arr_info(x, prec: int, device: Optional[int] = None):
+    """(pointer, nb, ld, host?, columns) of a band-major psi-like array."""
     want = np.complex128 if prec == _lib.EFFM_PREC_FP64 else np.complex64
     if _is_torch(x):
         import torch
@@ -42,21 +42,44 @@ def _as_2d_info(x, prec: int):
             raise TypeError(f"expected {tdt}, got {x.dtype}")
         if not x.is_cuda:
             raise TypeError("torch tensors must live on the GPU; pass numpy arrays for host data")
-        if x.dim() == 1:
-            x2 = x.view(1, -1)
-        else:
-            x2 = x
+        if device is not None and x.device.index != device:
+            raise ValueError(f"tensor is on cuda:{x.device.index}, the Hamiltonian on cuda:{device}")
+        if x.dim() not in (1, 2):
+            raise ValueError("psi-like arrays are [npw] or [nb, npw]")
+        x2 = x.view(1, -1) if x.dim() == 1 else x
         if x2.stride(-1) != 1:
             raise ValueError("last dimension must be contiguous")
-        return x.data_ptr(), x2.shape[0], x2.stride(0) if x2.shape[0] > 1 else x2.shape[1], False
+        ld = x2.stride(0) if x2.shape[0] > 1 else x2.shape[1]
+        return x.data_ptr(), x2.shape[0], ld, False, x2.shape[1]
     if isinstance(x, np.ndarray):
         if x.dtype != want:
             raise TypeError(f"expected {np.dtype(want)}, got {x.dtype}")
         if not x.flags.c_contiguous:
             raise ValueError("host arrays must be C-contiguous")
+        if x.ndim not in (1, 2):
+            raise ValueError("psi-like arrays are [npw] or [nb, npw]")
         x2 = x.reshape(1, -1) if x.ndim == 1 else x
-        return x.ctypes.data, x2.shape[0], x2.shape[1], True
+        return x.ctypes.data, x2.shape[0], x2.shape[1], True, x2.shape[1]
     raise TypeError(f"unsupported array type {type(x)}")
+
+
+def _bands(n_bands: Optional[int], dim: int, psi_info, out_infos) -> int:
+    """Band count of a call, checked against every array before the library
+    sees a pointer: each holds at least that many bands of at least dim()
+    coefficients (the kernels read and write nb rows of npw elements)."""
+    nb = psi_info[1]
+    nbo = min((o[1] for o in out_infos), default=nb)
+    if n_bands is None:
+        if nbo < nb:
+            raise ValueError(f"output holds {nbo} bands, psi {nb}")
+    else:
+        if not (0 <= n_bands <= min(nb, nbo)):
+            raise ValueError(f"n_bands={n_bands} outside [0, {min(nb, nbo)}] (bands held by psi / outputs)")
+        nb = n_bands
+    for info, what in [(psi_info, "psi")] + [(o, "output") for o in out_infos]:
+        if info[4] < dim:
+            raise ValueError(f"{what} has {info[4]} coefficients per band, the k-point needs {dim}")
+    return nb
 
 
 @dataclass
@@ -171,9 +194,14 @@ class Hamiltonian:
     def rayleigh(self, psi, hpsi, prec: str = "fp64"):
         """FP64 E_b = Re<psi_b|hpsi_b>/<psi_b|psi_b> (test_hpsi_eigen.cpp:49-56)."""
         p = PREC[prec]
-        pp, nb, ld, host = _as_2d_info(psi, p)
-        hp, nb2, ldh, host2 = _as_2d_info(hpsi, p)
-        npw = (psi.shape[-1])
+        pi = _arr_info(psi, p, self.device)
+        hi = _arr_info(hpsi, p, self.device)
+        pp, nb, ld, host, npw = pi
+        hp, nb2, ldh, host2, npw2 = hi
+        if host != host2:
+            raise TypeError("psi and hpsi must both be host or both be device arrays")
+        if nb2 < nb or npw2 < npw:
+            raise ValueError(f"hpsi [{nb2}, {npw2}] is smaller than psi [{nb}, {npw}]")
         if host:
             E = np.zeros(nb, dtype=np.float64)
             _lib.check(self._lib.effm_rayleigh(self._ctx, npw, nb, pp, ld, hp, ldh, p, E.ctypes.data,
@@ -196,7 +224,10 @@ class Hamiltonian:
         ops = [None if o is None else (C.c_void_p * nk)() for o in outs]
         nb = None
         for i in range(nk):
-            a, nbi, ld, host = _as_2d_info(psis[i], p)
+            a, nbi, ld, host, cols = _arr_info(psis[i], p, self.device)
+            npw = self._npw.get(ikpts[i])
+            if npw is not None and cols < npw:
+                raise ValueError(f"psi of k-point {ikpts[i]} has {cols} coefficients per band, it needs {npw}")
             if host:
                 raise ValueError("batched applies take device tensors")
             if nb is not None and nbi != nb:
@@ -206,8 +237,8 @@ class Hamiltonian:
             for o, arr in zip(outs, ops):
                 if o is None:
                     continue
-                b, nbo, ldo, host_o = _as_2d_info(o[i], p)
-                if host_o or ldo != ld or nbo < nbi:
+                b, nbo, ldo, host_o, ocols = _arr_info(o[i], p, self.device)
+                if host_o or ldo != ld or nbo < nbi or ocols < cols:
                     raise ValueError("outputs must be device tensors shaped like their psi")
                 arr[i] = b
         return nk, ks, nb or 0, pp, lds, ops
@@ -327,12 +358,14 @@ class Callable:
     def __call__(self, psi, hpsi, n_bands: Optional[int] = None, prec: str = "fp64",
                  overwrite: bool = False, graph: bool = False):
         p = PREC[prec]
-        pp, nb, ld, host = _as_2d_info(psi, p)
-        hp, nbo, ldo, host_o = _as_2d_info(hpsi, p)
+        dev = self._h.device
+        pi = _arr_info(psi, p, dev)
+        hi = _arr_info(hpsi, p, dev)
+        pp, _, ld, host, _ = pi
+        hp, _, ldo, host_o, _ = hi
         if host != host_o:
             raise TypeError("psi and hpsi must both be host or both be device arrays")
-        if n_bands is not None:
-            nb = n_bands
+        nb = _bands(n_bands, self.dim(), pi, [hi])
         flags = ((_lib.EFFM_F_OVERWRITE if overwrite else 0) | (_lib.EFFM_F_HOST if host else 0)
                  | (_lib.EFFM_F_GRAPH if graph else 0))
         if not host:
@@ -362,24 +395,31 @@ class GradientCallable:
         return _lib.check(self._h._lib.effm_kpt_dim(self._h._ctx, self._ikpt), self._h._ctx,
                           "Gradient::Callable::dim")
 
-    def __call__(self, psi, out: GradHPsi, prec: str = "fp64", overwrite: bool = False):
+    def __call__(self, psi, out: GradHPsi, prec: str = "fp64", overwrite: bool = False,
+                 n_bands: Optional[int] = None):
         p = PREC[prec]
-        pp, nb, ld, host = _as_2d_info(psi, p)
+        dev = self._h.device
+        pi = _arr_info(psi, p, dev)
+        pp, _, ld, host, _ = pi
         ptrs = []
+        infos = []
         ldo = None
         for o in (out.x, out.y, out.z):
             if o is None:
                 ptrs.append(None)
                 continue
-            op, nbo, l2, host_o = _as_2d_info(o, p)
+            oi = _arr_info(o, p, dev)
+            op, _, l2, host_o, _ = oi
             if host_o != host:
                 raise TypeError("psi and outputs must both be host or both be device arrays")
             if ldo is not None and l2 != ldo:
                 raise ValueError("all outputs need the same leading dimension")
             ldo = l2
             ptrs.append(op)
+            infos.append(oi)
         if ldo is None:
             return out
+        nb = _bands(n_bands, self.dim(), pi, infos)
         flags = (_lib.EFFM_F_OVERWRITE if overwrite else 0) | (_lib.EFFM_F_HOST if host else 0)
         if not host:
             self._h._sync_stream()

@@ -151,6 +151,8 @@ public:
             Callable() = default;
             void operator()(std::span<const std::complex<double>> psi, GradHPsi out) const {
                 const int ng = dim();
+                if (psi.size() < static_cast<size_t>(ng))
+                    throw std::invalid_argument("Gradient::Callable: psi span shorter than dim()");
                 auto ptr = [&](std::span<std::complex<double>> s) -> void* {
                     if (s.empty()) return nullptr;
                     if (s.size() < static_cast<size_t>(ng))

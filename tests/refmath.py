@@ -80,3 +80,36 @@ def dense_h(cell, kp, local=True, nonlocal_=True):
         if len(lam):
             H += (P.T * lam) @ P.conj()
     return H
+
+
+def dense_vnl_raw(cell, kp):
+    """V_NL = sum_atoms sum_{i,j: l_i = l_j} sum_m |p_im> D_ij <p_jm| from the
+    UNROTATED closed-form radial functions c q^l exp(-w q^2) of each raw
+    channel and the raw dion of make_species (D in Hartree), with the
+    cross-l entries of D dropped, as the reference's diagonalizeNonlocal
+    does (src/pseudo/ncpp_advance.cppm:65-153).  Independent of the
+    eigen-decomposition the engine consumes (lambda, rotated tables)."""
+    q, th, ph = spherical(kp.K)
+    norm = 1.0 / np.sqrt(cell.omega)
+    V = np.zeros((kp.npw, kp.npw), complex)
+    for s, sp in enumerate(cell.species_list):
+        r = sp.raw
+        ls, ws, c, D = r["ls"], r["w"], r["c"], r["dion"]
+        nb = len(ls)
+        for a in range(cell.natom):
+            if cell.species[a] != s:
+                continue
+            S = np.exp(-2j * np.pi * ((kp.gidx + kp.kfrac[None, :]) @ cell.tau[a]))
+            for i in range(nb):
+                for j in range(nb):
+                    if ls[i] != ls[j] or D[i, j] == 0.0:
+                        continue
+                    l = ls[i]
+                    bi = c * q ** l * np.exp(-ws[i] * q * q) * norm
+                    bj = c * q ** l * np.exp(-ws[j] * q * q) * norm
+                    for m in range(-l, l + 1):
+                        y = real_ylm(l, m, th, ph)
+                        pi = bi * y * S
+                        pj = bj * y * S
+                        V += D[i, j] * np.outer(pi, pj.conj())
+    return V

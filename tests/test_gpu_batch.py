@@ -254,3 +254,71 @@ def test_autotune_choices_keep_parity():
     ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0][:4], nthreads=8)
     assert band_rel(h[:4].cpu().numpy(), ref) <= 1e-12
     H.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_kgrid32_hpsi_dhdk_graph(prec):
+    """The 32-k-point batch at its own shape (64-atom cell, 4x4x2 grid,
+    ragged npw 36.8k-37.1k, two launch groups of 16 k-points; 4 nch = 2048
+    projection rows over several tiles per segment): effm_hpsi_dhdk_multi
+    replayed as a hipGraph equals, at every k-point, the single-k effm_hpsi
+    + effm_dhdk (FP64 1e-13; mixed 1e-5), and k-points 0 and 31 equal the
+    oracle (FP64 1e-12, mixed 2e-5).  A second replay after new psi contents
+    equals an eager call bit for bit.  hamiltonian_gradient.cpp:116-294."""
+    torch = _torch()
+    nb = 16
+    wl = workload("si64_kgrid", nb=nb)
+    nk = len(wl.kpts)
+    assert nk == 32 and len({kp.npw for kp in wl.kpts}) > 8
+    cdt = torch.complex128 if prec == "fp64" else torch.complex64
+    H = _load(wl)
+    ld = max(kp.npw for kp in wl.kpts)
+    # one ld for the whole batch (rows padded past npw), as a caller's pool
+    psis = []
+    for p in wl.psi:
+        t = torch.zeros((nb, ld), dtype=cdt, device="cuda")
+        t[:, :p.shape[1]] = torch.from_numpy(p).to(cdt)
+        psis.append(t)
+    hs = [torch.zeros_like(p) for p in psis]
+    gs = GradHPsi(*[[torch.zeros_like(p) for p in psis] for _ in range(3)])
+    ks = list(range(nk))
+    H.apply_with_gradient_multi(ks, psis, hs, gs, prec=prec, overwrite=True, graph=True)
+    torch.cuda.synchronize()
+    tol_single = 1e-13 if prec == "fp64" else 1e-5
+    worst = 0.0
+    for ik in ks:
+        npw = wl.kpts[ik].npw
+        sh = torch.zeros_like(psis[ik])
+        H.at_k(ik)(psis[ik], sh, prec=prec, overwrite=True)
+        sg = [torch.zeros_like(psis[ik]) for _ in range(3)]
+        H.gradient().at_k(ik)(psis[ik], GradHPsi(*sg), prec=prec, overwrite=True)
+        torch.cuda.synchronize()
+        pairs = [(hs[ik], sh)] + [(getattr(gs, d)[ik], sg[a]) for a, d in enumerate("xyz")]
+        for got, ref in pairs:
+            worst = max(worst, band_rel(got[:, :npw].cpu().numpy().astype(np.complex128),
+                                        ref[:, :npw].cpu().numpy().astype(np.complex128)))
+    log(f"{prec}: 32 k-points batch+graph vs single-k worst {worst:.2e}")
+    assert worst <= tol_single
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    for ik in (0, 31):
+        kp, psi = wl.kpts[ik], wl.psi[ik][:8]
+        ref = oracle.hpsi(wl.cell, kp, psi, nthreads=8)
+        gref = oracle.dhdk(wl.cell, kp, psi)
+        e = band_rel(hs[ik][:8, :kp.npw].cpu().numpy().astype(np.complex128), ref)
+        eg = max(band_rel(getattr(gs, d)[ik][:8, :kp.npw].cpu().numpy().astype(np.complex128), gref[a])
+                 for a, d in enumerate("xyz"))
+        log(f"{prec}: k {ik} vs oracle: H {e:.2e}, dH/dk {eg:.2e}")
+        assert e <= tol and eg <= tol
+    # new psi contents, same buffers: the replay equals an eager call
+    for p in psis:
+        p.mul_(complex(0.6, -0.8))
+    H.apply_with_gradient_multi(ks, psis, hs, gs, prec=prec, overwrite=True, graph=True)
+    eh = [torch.zeros_like(p) for p in psis]
+    eg_ = GradHPsi(*[[torch.zeros_like(p) for p in psis] for _ in range(3)])
+    H.apply_with_gradient_multi(ks, psis, eh, eg_, prec=prec, overwrite=True, graph=False)
+    torch.cuda.synchronize()
+    for ik in (0, 15, 16, 31):
+        assert torch.equal(hs[ik], eh[ik])
+        for d in "xyz":
+            assert torch.equal(getattr(gs, d)[ik], getattr(eg_, d)[ik])
+    H.close()

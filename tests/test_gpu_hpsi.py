@@ -148,13 +148,14 @@ def test_dhdk_fp64():
         e = band_rel(outs[a].cpu().numpy(), ref[a])
         log(f"dhdk dir {a} rel err {e:.2e}")
         assert e <= 1e-12
-    # skipped direction stays untouched
-    y = torch.full_like(psi, 7.0)
+    # a skipped middle direction (empty span, hamiltonian.cppm:53-65) does
+    # not shift the others: x and z still equal the oracle's
     x = torch.zeros_like(psi)
-    H.gradient().at_k(0)(psi, GradHPsi(x=x, z=None))
+    z = torch.zeros_like(psi)
+    H.gradient().at_k(0)(psi, GradHPsi(x=x, y=None, z=z))
     torch.cuda.synchronize()
     assert band_rel(x.cpu().numpy(), ref[0]) <= 1e-12
-    assert (y.cpu().numpy() == 7.0).all()
+    assert band_rel(z.cpu().numpy(), ref[2]) <= 1e-12
     H.close()
 
 
@@ -223,6 +224,15 @@ def test_error_paths():
         assert band_rel(h.cpu().numpy(), oracle.hpsi(wl.cell, kp, wl.psi[0])) <= 1e-12
     finally:
         Hs.close()
+    # an unknown precision is refused, not read as complex64
+    Hr = Hamiltonian(wl.cell)
+    try:
+        lib = _lib.load()
+        E = torch.zeros(2, dtype=torch.float64, device="cuda")
+        rc = lib.effm_rayleigh(Hr.ctx, kp.npw, 2, p.data_ptr(), kp.npw, p.data_ptr(), kp.npw, 7, E.data_ptr(), 0)
+        assert rc == _lib.EFFM_ERR_INVALID
+    finally:
+        Hr.close()
     # ... and no room for one band's planes fails loudly
     with pytest.raises(_lib.EffmOutOfMemory):
         Hs = Hamiltonian(wl.cell, hbm_budget=resident + proj + nplanes * plane_bytes // 4)
@@ -340,3 +350,81 @@ def test_odd_sizes(grid, ecut, nb):
         assert band_rel(outs[a].cpu().numpy(), ref[a]) <= 1e-12
         assert band_rel(outs32[a].cpu().numpy(), ref[a]) <= 2e-5
     H.close()
+
+
+def _raw_dion_wl(ls, seed, widths=(0.25, 0.5), nb=6):
+    """2-atom fcc cell, off Gamma, whose species has a random symmetric dion
+    over all channels (non-diagonal l-blocks, non-zero cross-l entries)."""
+    sp = synth.make_species(synth.SplitMix64(seed), ls=ls, widths=widths)
+    return synth.custom(*synth.fcc_primitive(10.26), (36, 36, 36), 30.0, [np.array([0.15, -0.05, 0.2])], nb,
+                        seed=seed + 1, species_list=[sp])
+
+
+def _h_and_grad(wl, prec):
+    torch = _torch()
+    H = Hamiltonian(wl.cell)
+    H.load_kpoint(0, wl.kpts[0])
+    psi = wl.psi[0].astype(np.complex64 if prec == "mixed" else np.complex128)
+    p = torch.from_numpy(psi).cuda()
+    h = torch.zeros_like(p)
+    H.at_k(0)(p, h, prec=prec)
+    g = [torch.zeros_like(p) for _ in range(3)]
+    H.gradient().at_k(0)(p, GradHPsi(*g), prec=prec)
+    torch.cuda.synchronize()
+    out = h.cpu().numpy(), [x.cpu().numpy() for x in g]
+    H.close()
+    return out
+
+
+@pytest.mark.parametrize("ls", [(0, 1, 2, 3), (0, 0, 1, 1, 2, 2)])
+def test_high_l_projectors(ls):
+    """Projectors up to l = 3 on the device: the three-term Legendre advance,
+    its dtheta form and the m >= 2 gradient seeds of the device Y_lm
+    (k_nonlocal.hip ylm_eval) against the oracle; FP64 H|psi> and dH/dk at
+    1e-12, mixed at 2e-5."""
+    wl = _raw_dion_wl(ls, seed=91 + len(ls), widths=(0.3, 0.45))
+    ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0])
+    gref = oracle.dhdk(wl.cell, wl.kpts[0], wl.psi[0])
+    for prec, tol in (("fp64", 1e-12), ("mixed", 2e-5)):
+        h, g = _h_and_grad(wl, prec)
+        e = band_rel(h, ref)
+        eg = max(band_rel(g[a], gref[a]) for a in range(3))
+        log(f"ls {ls} {prec}: H {e:.2e}, dH/dk {eg:.2e}")
+        assert e <= tol and eg <= tol
+
+
+def test_raw_dion_dense_on_device():
+    """The device H with a non-diagonal dion (cross-l entries included)
+    equals the dense H whose nonlocal part is sum_ij |p_i> D_ij <p_j| over
+    the UNROTATED closed-form projectors with cross-l D dropped
+    (src/pseudo/ncpp_advance.cppm:65-153), at the table-interpolation
+    tolerance."""
+    import refmath
+    wl = _raw_dion_wl((0, 0, 1, 1, 2), seed=57)
+    kp = wl.kpts[0]
+    Hd = refmath.dense_h(wl.cell, kp, nonlocal_=False) + refmath.dense_vnl_raw(wl.cell, kp)
+    ref = wl.psi[0] @ Hd.T
+    h, _ = _h_and_grad(wl, "fp64")
+    e = band_rel(h, ref)
+    log(f"raw dion dense pin {e:.2e}")
+    assert e <= 1e-8
+
+
+def test_ge_d_projector_upf():
+    """The reference's norm-conserving Ge-spd-high.PD04 UPF (l = 0, 1, 2
+    projector pairs; test/data_upf) on a 2-atom Ge fcc cell: FP64 H|psi> and
+    dH/dk equal the oracle at 1e-12, mixed at 2e-5."""
+    import os
+    from effmass_amd import pwmat
+    g = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pwmat")
+    sp = pwmat.species_from_upf(pwmat.read_upf(os.path.join(g, "Ge-spd-high.PD04.PBE.UPF")), 32)
+    wl = synth.custom(*synth.fcc_primitive(10.69), (36, 36, 36), 30.0, [np.array([0.25, 0.1, -0.15])], 6,
+                      seed=32, species_list=[sp])
+    ref = oracle.hpsi(wl.cell, wl.kpts[0], wl.psi[0])
+    gref = oracle.dhdk(wl.cell, wl.kpts[0], wl.psi[0])
+    for prec, tol in (("fp64", 1e-12), ("mixed", 2e-5)):
+        h, gr = _h_and_grad(wl, prec)
+        e = band_rel(h, ref)
+        eg = max(band_rel(gr[a], gref[a]) for a in range(3))
+        log(f"Ge PD04 {prec}: H {e:.2e}, dH/dk {eg:.2e}")
+        assert e <= tol and eg <= tol

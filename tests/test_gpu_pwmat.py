@@ -2,10 +2,17 @@
 reference's test data by tests/golden/make_pwmat_fixtures.py):
 
 * FP64 device H|psi> equals the oracle on the 8-atom Si and 4-atom AlN cells
-  (real UPF projectors, two species, a 20x20x32 grid);
+  (real UPF projectors, two species, a 20x20x32 grid, a GKK on 2 nodes);
 * the lowest eigenvalues of the device H (LOBPCG, FP64 Rayleigh-Ritz)
-  match PWmat's OUT.EIGEN within the reference's own tolerance of
-  2e-5 Ha (test/H_psi/test_hpsi_eigen.cpp:37)."""
+  match PWmat's OUT.EIGEN at EVERY k-point of AlN (10) and of Si OUT1 / OUT2
+  (7 each) within the reference's own tolerance of 2e-5 Ha
+  (test/H_psi/test_hpsi_eigen.cpp:37);
+* Hellmann-Feynman <psi|dH/dk|psi> of the device gradient equals the
+  central finite difference of OUT1.EIGEN along [100], and the [010] /
+  [001] k-points reproduce the [100] slope, with the band selection and
+  tolerances of test/H_psi/test_hgrad_eigen_fd.cpp:50-60, 225-266, 346-399.
+  The reference's OUT1.WG is absent, so the eigenvectors are the device
+  H's own (LOBPCG)."""
 import os
 
 import numpy as np
@@ -14,28 +21,38 @@ import torch
 
 import oracle
 from effmass_amd import eigensolver, pwmat
-from effmass_amd.hamiltonian import Hamiltonian
+from effmass_amd.hamiltonian import GradHPsi, Hamiltonian
 
 pytestmark = pytest.mark.gpu
 G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pwmat")
 CASES = {
-    "si8": ("si8_atom.config", "si8_IN.VR", {14: "Si.SG15.PBE.UPF"}, "si8_OUT2_k0.GKK", "si8_OUT2.EIGEN"),
+    "si8": ("si8_atom.config", "si8_IN.VR", {14: "Si.SG15.PBE.UPF"}, "si8_OUT2.GKK", "si8_OUT2.EIGEN"),
+    "si8_out1": ("si8_atom.config", "si8_IN.VR", {14: "Si.SG15.PBE.UPF"}, "si8_OUT1.GKK", "si8_OUT1.EIGEN"),
     "aln": ("aln_atom.config", "aln_OUT.VR", {7: "N.SG15.PBE.UPF", 13: "Al.SG15.PBE.UPF"},
-            "aln_OUT_k0.GKK", "aln_OUT.EIGEN"),
+            "aln_OUT.GKK", "aln_OUT.EIGEN"),
 }
+_CACHE = {}
 
 
 def load(case):
-    ac, vr, upfs, gkk, eig = CASES[case]
-    g = pwmat.GKKFile(os.path.join(G, gkk))
-    cell = pwmat.load_cell(os.path.join(G, ac), os.path.join(G, vr),
-                           {z: os.path.join(G, f) for z, f in upfs.items()}, n=g.meta.n)
-    return cell, g.kpoint(0), pwmat.read_eigen(os.path.join(G, eig))["eig"][0, 0]
+    """(cell, GKK file, EIGEN dict) of a fixture case."""
+    if case not in _CACHE:
+        ac, vr, upfs, gkk, eig = CASES[case]
+        g = pwmat.GKKFile(os.path.join(G, gkk))
+        cell = pwmat.load_cell(os.path.join(G, ac), os.path.join(G, vr),
+                               {z: os.path.join(G, f) for z, f in upfs.items()}, n=g.meta.n)
+        _CACHE[case] = (cell, g, pwmat.read_eigen(os.path.join(G, eig)))
+    return _CACHE[case]
 
 
-@pytest.mark.parametrize("case", ["si8", "aln"])
+def log(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("case", ["si8", "aln", "si8_out1"])
 def test_fixture_cell_parity(case):
-    cell, kp, _ = load(case)
+    cell, g, _ = load(case)
+    kp = g.kpoint(0)
     H = Hamiltonian(cell)
     H.load_kpoint(0, kp)
     psi = eigensolver.random_start(6, kp.npw, torch.from_numpy(kp.kin), seed=4).numpy()
@@ -46,25 +63,87 @@ def test_fixture_cell_parity(case):
     ref = oracle.hpsi(cell, kp, psi)
     got = h.cpu().numpy()
     e = (np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)).max()
-    print(f"{case} fixture parity {e:.2e}", flush=True)
+    log(f"{case} fixture parity {e:.2e}")
     assert e <= 1e-12
     H.close()
 
 
-@pytest.mark.parametrize("case,nev", [("si8", 26), ("aln", 20)])
-def test_eigenvalues_vs_pwmat(case, nev):
-    cell, kp, ref = load(case)
-    H = Hamiltonian(cell)
-    H.load_kpoint(0, kp)
+def solve(H, ik, kp, nev, nb=None, seed=7):
     kin = torch.from_numpy(kp.kin).cuda()
-    nb = nev + 8
-    X0 = eigensolver.random_start(nb, kp.npw, kin, seed=7, device="cuda")
-    op = H.at_k(0)
+    nb = nb or nev + 8
+    X0 = eigensolver.random_start(nb, kp.npw, kin, seed=seed, device="cuda")
+    op = H.at_k(ik)
     res = eigensolver.lobpcg(lambda V, HV: op(V, HV, overwrite=True), X0, kin, nconv=nev, tol=1e-6,
                              maxiter=400)
-    E = res.E[:nev].cpu().numpy()
-    dE = np.abs(E - ref[:nev])
-    print(f"{case}: {res.iters} iterations, max |E - E_PWmat| = {dE.max():.2e} Ha", flush=True)
     assert res.converged == nev
-    assert dE.max() < 2e-5
+    return res
+
+
+@pytest.mark.parametrize("case,nev", [("si8", 26), ("aln", 20), ("si8_out1", 26)])
+def test_eigenvalues_vs_pwmat_every_kpoint(case, nev):
+    """test_hpsi_eigen.cpp:173-181, 293-301 check every k-point of the file;
+    so does this, at the same 2e-5 Ha."""
+    cell, g, eig = load(case)
+    H = Hamiltonian(cell)
+    worst = 0.0
+    for ik in range(g.nkpt):
+        kp = g.kpoint(ik)
+        H.load_kpoint(ik, kp)
+        res = solve(H, ik, kp, nev)
+        dE = np.abs(res.E[:nev].cpu().numpy() - eig["eig"][0, ik, :nev]).max()
+        log(f"{case} k {ik}: npw {kp.npw}, {res.iters} iterations, max |E - E_PWmat| = {dE:.2e} Ha")
+        worst = max(worst, dE)
+        H.release_kpoint(ik)
+    assert worst < 2e-5
     H.close()
+
+
+# test_hgrad_eigen_fd.cpp:50-53, 65-70
+TOL_FD_HF, TOL_ZERO, TOL_IMAG = 2.0e-3, 1.0e-4, 1.0e-5
+HF_BANDS = (1, 6, 13, 16, 17, 22)
+
+
+def _hf(H, ik, X):
+    """<psi_n|dH/dk_a|psi_n> / <psi_n|psi_n> for a = x, y, z (complex)."""
+    outs = [torch.zeros_like(X) for _ in range(3)]
+    H.gradient().at_k(ik)(X, GradHPsi(*outs), overwrite=True)
+    nrm = (X.conj() * X).real.sum(1)
+    return np.stack([((X.conj() * o).sum(1) / nrm).cpu().numpy() for o in outs])
+
+
+def test_hellmann_feynman_vs_eigen_fd():
+    cell, g, eig = load("si8_out1")
+    assert g.meta.nnode == 2 and g.nkpt == 7
+    E, kv = eig["eig"][0], eig["kvec"]
+    H = Hamiltonian(cell)
+    hf = {}
+    for ik in range(7):
+        kp = g.kpoint(ik)
+        H.load_kpoint(ik, kp)
+        res = solve(H, ik, kp, nev=24, nb=32, seed=11 + ik)
+        dE = np.abs(res.E[:24].cpu().numpy() - E[ik, :24]).max()
+        assert dE < 2e-5, (ik, dE)
+        hf[ik] = _hf(H, ik, res.X[:24].contiguous())
+        H.release_kpoint(ik)
+    H.close()
+    worst = [0.0, 0.0, 0.0]
+    # interior k-points of the [100] line: central differences (:225-266)
+    for ik in (1, 2, 3):
+        for n in HF_BANDS:
+            fd = (E[ik + 1, n] - E[ik - 1, n]) / (kv[ik + 1, 0] - kv[ik - 1, 0])
+            h = hf[ik][:, n]
+            log(f"k {ik} band {n:2d}: FD_x {fd:+.5f} HF_x {h[0].real:+.5f} HF_y {h[1].real:+.1e} "
+                f"HF_z {h[2].real:+.1e} Im {h[0].imag:+.1e}")
+            worst = [max(worst[0], abs(h[0].real - fd)), max(worst[1], abs(h[1].real), abs(h[2].real)),
+                     max(worst[2], abs(h[0].imag))]
+    # [010] and [001] points equal the [100] slope at k-point 1 (:346-399)
+    for ik, d in ((5, 1), (6, 2)):
+        for n in HF_BANDS:
+            fd1 = (E[2, n] - E[0, n]) / (kv[2, 0] - kv[0, 0])
+            h = hf[ik][:, n]
+            cross = [h[a].real for a in range(3) if a != d]
+            log(f"k {ik} band {n:2d}: HF_dir {h[d].real:+.5f} vs FD_x(k1) {fd1:+.5f}, cross {cross}")
+            worst = [max(worst[0], abs(h[d].real - fd1)), max(worst[1], *map(abs, cross)),
+                     max(worst[2], abs(h[d].imag))]
+    log(f"max |HF - FD| {worst[0]:.2e}, max |cross| {worst[1]:.2e}, max |Im| {worst[2]:.2e} (Ha bohr)")
+    assert worst[0] < TOL_FD_HF and worst[1] < TOL_ZERO and worst[2] < TOL_IMAG

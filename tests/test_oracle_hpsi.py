@@ -123,3 +123,51 @@ def test_dhdk_kinetic_only_and_skip():
     x, y, z = oracle.dhdk(cell, kp, psi, dirs=(True, False, True))
     assert y is None
     assert np.abs(x - kp.K[:, 0] * psi).max() == 0 and np.abs(z - kp.K[:, 2] * psi).max() == 0
+
+
+def _raw_dion_cell(ls, seed, widths=(0.25, 0.5)):
+    """2-atom fcc cell whose species has a random symmetric dion over all
+    channels: non-diagonal l-blocks and non-zero cross-l entries."""
+    rng = synth.SplitMix64(seed)
+    sp = synth.make_species(rng, ls=ls, widths=widths)
+    wl = synth.custom(*synth.fcc_primitive(10.26), (36, 36, 36), 30.0, [np.array([0.15, -0.05, 0.2])], 3,
+                      seed=seed + 1, species_list=[sp])
+    return wl
+
+
+@pytest.mark.parametrize("ls", [(0, 0, 1, 1), (0, 0, 1, 1, 2, 2), (0, 1, 1, 2, 3)])
+def test_nonlocal_raw_dion_vs_dense(ls):
+    """The lambda / rotated-beta form the engine consumes equals the dense
+    sum_ij |p_i> D_ij <p_j| of the unrotated projectors, cross-l D dropped
+    (interpolation tolerance).  A wrong rotation (U for U^T, a missed
+    eigenvector) or a kept cross-l block fails."""
+    wl = _raw_dion_cell(ls, seed=31 + len(ls))
+    cell, kp, psi = wl.cell, wl.kpts[0], wl.psi[0]
+    V = refmath.dense_vnl_raw(cell, kp)
+    got = oracle.hpsi(cell, kp, psi, nonlocal_only=True)
+    ref = psi @ V.T
+    assert _rel(got, ref) < 1e-8
+    # the dropped cross-l entries are not small: D_ij ~ U(-1, 1) Ry
+    lsa = np.asarray(ls)
+    D = cell.species_list[0].raw["dion"]
+    assert np.abs(D[lsa[:, None] != lsa[None, :]]).max() > 0.05
+    # and the l-blocks are far from diagonal, so the rotation matters
+    for l in set(ls):
+        idx = np.nonzero(lsa == l)[0]
+        if len(idx) > 1:
+            blk = D[np.ix_(idx, idx)]
+            assert np.abs(blk - np.diag(np.diag(blk))).max() > 0.01
+
+
+def test_high_l_vs_dense():
+    """l up to 3 (three-term Legendre advance, m >= 2 seeds) in H and dH/dk
+    against the dense closed forms with scipy's Y_lm."""
+    wl = _raw_dion_cell((0, 1, 2, 3), seed=77, widths=(0.3,))
+    cell, kp, psi = wl.cell, wl.kpts[0], wl.psi[0]
+    H = refmath.dense_h(cell, kp)
+    assert _rel(oracle.hpsi(cell, kp, psi), psi @ H.T) < 1e-8
+    P, lam, dP = refmath.projectors(cell, kp, deriv=True)
+    g = oracle.dhdk(cell, kp, psi)
+    for al in range(3):
+        D = np.diag(kp.K[:, al]) + (dP[:, al].T * lam) @ P.conj() + (P.T * lam) @ dP[:, al].conj()
+        assert _rel(g[al], psi @ D.T) < 1e-7, al

@@ -95,3 +95,28 @@ def test_sfac():
         s = oracle.sfac(tau, g, kf, n)
         assert abs(abs(s) - 1) < 1e-14
         assert abs(s - np.exp(-2j * np.pi * np.dot(g + kf, tau))) < 1e-13
+
+
+def test_interp_stencil_centring():
+    """Quartic node data tells the stencil apart: a cubic Lagrange
+    interpolant through nodes j_0..j_3 misses f(x) = (x - c)^4 (x = q/dq in
+    node units) by exactly prod_k (x - j_k), since the fourth derivative is
+    the constant 4!.  The reference centres the stencil on
+    i0 = floor(q/dq) - 1, clamped to 0 below the second node
+    (src/pseudo/ncpp_advance.cppm:663-673); an uncentred stencil
+    i0 = floor(q/dq) has another product and fails."""
+    dq, n = 0.01, 2004
+    qmax = (n - 4) * dq
+    for q in (0.0037, 0.0123, 0.0251, 0.5043, 1.234567, 7.77777, 19.5071):
+        s = q / dq
+        c = math.floor(s)
+        t = (np.arange(n, dtype=np.float64) - c) ** 4
+        i0 = max(c - 1, 0)
+        want = (s - c) ** 4 - np.prod(s - (i0 + np.arange(4)))
+        rc, got = oracle.interp(t, dq, qmax, 1.0, q)
+        assert rc == 0
+        assert abs(got - want) <= 1e-9, q
+        if c >= 1:
+            # past the first node the uncentred choice misses by O(0.1)
+            off = (s - c) ** 4 - np.prod(s - (c + np.arange(4)))
+            assert abs(off - want) > 1e-2, q

@@ -13,8 +13,8 @@ G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pwmat")
 
 
 def test_gkk_golden_values():
-    g = pwmat.GKKFile(os.path.join(G, "aln_OUT_k0.GKK"))
-    assert g.meta.n == (20, 20, 32) and g.meta.nnode == 1
+    g = pwmat.GKKFile(os.path.join(G, "aln_OUT.GKK"))
+    assert g.meta.n == (20, 20, 32) and g.meta.nnode == 1 and g.nkpt == 10
     kp = g.kpoint(0)
     assert kp.npw == 1737
     assert abs(kp.kin[0] - 0.7529635525206351) < 1e-12
@@ -29,16 +29,69 @@ def test_gkk_golden_values():
     B = 2 * np.pi * np.linalg.inv(g.meta.A).T
     assert np.abs((kp.gidx + kp.kfrac) @ B - kp.K).max() < 1e-10
     with pytest.raises(IndexError):
-        g.kpoint(1)
+        g.kpoint(10)
+    with pytest.raises(IndexError):
+        g.kpoint(-1)
 
 
 def test_gkk_kpoint_inference_vs_eigen():
-    g = pwmat.GKKFile(os.path.join(G, "si8_OUT2_k0.GKK"))
-    kp = g.kpoint(0)
-    assert kp.npw == 6451 and g.meta.ecut == 25.0
-    e = pwmat.read_eigen(os.path.join(G, "si8_OUT2.EIGEN"))
+    for gkk, eig in (("si8_OUT2.GKK", "si8_OUT2.EIGEN"), ("si8_OUT1.GKK", "si8_OUT1.EIGEN"),
+                     ("aln_OUT.GKK", "aln_OUT.EIGEN")):
+        g = pwmat.GKKFile(os.path.join(G, gkk))
+        e = pwmat.read_eigen(os.path.join(G, eig))
+        assert e["nkpt"] == g.nkpt
+        B = 2 * np.pi * np.linalg.inv(g.meta.A).T
+        for ik in range(g.nkpt):
+            kp = g.kpoint(ik)
+            assert np.abs(kp.kfrac @ B - e["kvec"][ik]).max() < 1e-10, (gkk, ik)
+    assert pwmat.GKKFile(os.path.join(G, "si8_OUT2.GKK")).kpoint(0).npw == 6451
+
+
+def _raw_records(path):
+    import struct
+    buf = open(path, "rb").read()
+    o, out = 0, []
+    while o < len(buf):
+        (n,) = struct.unpack_from("<i", buf, o)
+        out.append(buf[o + 4:o + 4 + n])
+        o += 8 + n
+    return out
+
+
+def test_gkk_two_node_merge():
+    """OUT1.GKK holds each k-point on 2 nodes (test/data_nonscf, the file
+    the reference's gradient test loads).  loadKPoint concatenates the
+    nodes' first ngtotnod entries in node order and negates the stored
+    -(k+G) (src/io/GKK.cppm:525-549): npw is the sum over nodes, and the
+    merged G-set is the whole sphere |k+G|^2 <= Ecut, each G once."""
+    path = os.path.join(G, "si8_OUT1.GKK")
+    g = pwmat.GKKFile(path)
+    assert g.meta.nnode == 2 and g.nkpt == 7 and g.meta.n == (36, 36, 36)
+    assert g.ngtotnod[0].tolist() == [3245, 3222]
+    recs = _raw_records(path)
     B = 2 * np.pi * np.linalg.inv(g.meta.A).T
-    assert np.abs(kp.kfrac @ B - e["kvec"][0]).max() < 1e-10
+    r = 4                                            # header, Ecut, AL, ngtotnod
+    for ik in range(g.nkpt):
+        kp = g.kpoint(ik)
+        ng = g.ngtotnod[ik]
+        assert kp.npw == int(ng.sum())
+        parts = []
+        for inode in range(2):
+            m = int(ng[inode])
+            kin, x, y, z = (np.frombuffer(recs[r + a], "<f8", count=m) for a in range(4))
+            parts.append((kin, -np.stack([x, y, z], 1)))
+            r += 4
+        assert np.array_equal(kp.kin, np.concatenate([p[0] for p in parts]))
+        assert np.array_equal(kp.K, np.concatenate([p[1] for p in parts]))
+        # the merged set is the full sphere, each integer G exactly once
+        assert len({tuple(v) for v in kp.gidx.tolist()}) == kp.npw
+        ax = [np.arange(-18, 18)] * 3
+        gg = np.stack(np.meshgrid(*ax, indexing="ij"), -1).reshape(-1, 3)
+        k2 = (((gg + kp.kfrac) @ B) ** 2).sum(1)
+        # the header's Ecut bounds the kinetic energy |K|^2 / 2 (Hartree)
+        inside = int((0.5 * k2 <= g.meta.ecut * (1 + 1e-9)).sum())
+        assert inside == kp.npw, (ik, inside, kp.npw)
+    assert r == len(recs)
 
 
 def test_vr_golden_values():
@@ -55,6 +108,24 @@ def test_eigen_and_atom_config():
     assert list(Z) == [7, 7, 13, 13]                 # ascending Z, stable (ATOM.cppm:290-316)
     A8, Z8, f8 = pwmat.read_atom_config(os.path.join(G, "si8_atom.config"))
     assert np.allclose(A8, np.eye(3) * 5.43102018 / 0.52917721067)
+
+
+def test_upf_d_projectors():
+    """Ge-spd-high.PD04 (test/data_upf): norm-conserving with l = 0, 1, 2
+    projector pairs; sortByL keeps them, the l = 2 tables vanish like q^2."""
+    u = pwmat.read_upf(os.path.join(G, "Ge-spd-high.PD04.PBE.UPF"))
+    assert u["pseudo_type"] == "NC" and list(u["lll"]) == [0, 0, 1, 1, 2, 2]
+    sp = pwmat.species_from_upf(u, 32)
+    assert sp.lmax == 2 and list(sp.beta_l) == [0, 0, 1, 1, 2, 2]
+    D = u["dion_ry"] / 2
+    for l in (0, 1, 2):
+        idx = np.nonzero(u["lll"] == l)[0]
+        assert np.allclose(np.sort(sp.lam[sp.beta_l == l]), np.linalg.eigvalsh(D[np.ix_(idx, idx)]))
+    d = sp.beta_tab[sp.beta_l == 2]
+    q = np.arange(sp.nq) * sp.dq
+    assert np.all(d[:, 0] == 0.0)
+    ratio = d[:, 1:6] / q[1:6] ** 2                    # beta_2(q) ~ q^2 near 0
+    assert np.abs(ratio / ratio[:, :1] - 1).max() < 1e-3
 
 
 def test_upf_and_tables():
