@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "effm_hpsi.h"
@@ -20,6 +21,46 @@ struct effm_ctx {
 };
 
 namespace effm {
+
+// ---------------------------------------------------------------------------
+//  Static scratch of effm_overlap_noalloc: the split-K partial sums of an
+//  overlap, at most kOverlapPartialCap bytes, in a device array that every
+//  GPU the library runs on holds until the process ends.  In host code
+//  `scratch` names a host-side stand-in, not the device memory: the device
+//  address comes from hipGetSymbolAddress, once per device with that device
+//  current (at effm_ctx_create), and only that address is passed to kernels.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kOverlapPartialCap];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static hipError_t g_scratch_rc[kMaxDevices] = {};
+static bool g_scratch_done[kMaxDevices] = {};
+
+// The calling thread has `device` current.
+static void scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_rc[device] = e;
+    g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+    g_scratch_done[device] = true;
+}
+
+// Device address of this device's scratch, or NULL (with the reason in c.err).
+static void* scratch_ptr(Ctx& c) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (c.device < 0 || c.device >= kMaxDevices || !g_scratch_done[c.device]) {
+        c.err = "static scratch not set up for this device";
+        return nullptr;
+    }
+    if (!g_scratch_ptr[c.device])
+        c.err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(g_scratch_rc[c.device]);
+    return g_scratch_ptr[c.device];
+}
 
 // ---------------------------------------------------------------------------
 //  small helpers
@@ -522,6 +563,8 @@ int effm_ctx_create(effm_ctx** out, int device, uint64_t hbm_budget) {
         h->c.ncu = ncu;
     h->c.budget = hbm_budget;
     h->c.stream = nullptr;   // legacy default stream until effm_set_stream
+    scratch_init(device);
+    prime_occupancy(h->c);
     *out = h;
     return kOk;
 }
@@ -1023,6 +1066,37 @@ int effm_overlap(effm_ctx* h, int m, int n, int k, const void* A, int64_t lda, c
                        ? launch_overlap<double>(c, m, n, k, A, lda, B, ldb, C, ldc, c.partial.ptr, c.partial.bytes)
                        : launch_overlap<float>(c, m, n, k, A, lda, B, ldb, C, ldc, c.partial.ptr, c.partial.bytes);
     if (e != hipSuccess) return hip_fail(c, e, "overlap");
+    return kOk;
+}
+
+int effm_overlap_noalloc(effm_ctx* h, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
+                         void* C, int64_t ldc, int prec, int flags) {
+    if (!h) return kErrInvalid;
+    Ctx& c = h->c;
+    (void)hipSetDevice(c.device);
+    if (m < 0 || n < 0 || k < 0) return fail(c, kErrInvalid, "negative overlap dimension");
+    if (m == 0 || n == 0) return kOk;
+    if (!C || (k > 0 && (!A || !B))) return fail(c, kErrInvalid, "overlap operands must not be NULL");
+    if (k > 0 && (lda < k || ldb < k)) return fail(c, kErrInvalid, "leading dimension smaller than k");
+    if (ldc < n) return fail(c, kErrInvalid, "ldc smaller than n");
+    if (prec != EFFM_PREC_FP64 && prec != EFFM_PREC_MIXED) return fail(c, kErrInvalid, "unknown precision");
+    if (flags & (EFFM_F_HOST | EFFM_F_GRAPH))
+        return fail(c, kErrInvalid, "effm_overlap_noalloc takes device pointers only");
+    // the same split count as effm_overlap, so the same partial sums
+    const size_t pb = prec == EFFM_PREC_FP64 ? overlap_partial_bytes<double>(c, m, n, std::max(k, 1))
+                                             : overlap_partial_bytes<float>(c, m, n, std::max(k, 1));
+    if (pb > kOverlapPartialCap)
+        return fail(c, kErrInvalid, "effm_overlap_noalloc: an m x n block of partial sums (" + std::to_string(pb) +
+                                        " B) exceeds the " + std::to_string(kOverlapPartialCap >> 20) +
+                                        " MiB static scratch");
+    void* part = scratch_ptr(c);
+    if (!part) return kErrHip;
+    // every partial sum is written by the GEMM before the reduction reads it,
+    // so the scratch needs no clearing between calls
+    hipError_t e = prec == EFFM_PREC_FP64
+                       ? launch_overlap<double>(c, m, n, k, A, lda, B, ldb, C, ldc, part, kOverlapPartialCap, false)
+                       : launch_overlap<float>(c, m, n, k, A, lda, B, ldb, C, ldc, part, kOverlapPartialCap, false);
+    if (e != hipSuccess) return hip_fail(c, e, "overlap (no allocation)");
     return kOk;
 }
 

@@ -258,12 +258,16 @@ hipError_t launch_dhdk(Ctx& c, const ApplySeg* segs, int nseg, int nb, bool accu
 template <typename T>
 size_t vnl_partial_bytes(Ctx& c, int nrows, int nb, int min_npw, int nseg);
 
-// overlap C = conj(A) B^T with FP64-reduced split-K partial sums
+// overlap C = conj(A) B^T with FP64-reduced split-K partial sums; the split
+// count keeps the partial sums within kOverlapPartialCap bytes whenever one
+// split's m*n block fits (an eigensolver's context runs under a small budget)
+constexpr size_t kOverlapPartialCap = size_t(32) << 20;
 template <typename T>
 size_t overlap_partial_bytes(Ctx& c, int m, int n, int k);
 template <typename T>
 hipError_t launch_overlap(Ctx& c, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
-                          void* C, int64_t ldc, void* partial, size_t partial_bytes);
+                          void* C, int64_t ldc, void* partial, size_t partial_bytes, bool timed = true);
+void prime_occupancy(Ctx& c);
 
 // candidate launch shapes effm_autotune times
 constexpr int kPencilBandChoices[] = {4, 8, 16};

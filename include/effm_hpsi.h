@@ -162,6 +162,20 @@ int effm_hpsi_dhdk_multi(effm_ctx* ctx, int nk, const int* ikpts, int nb,
                          void* const* out_y, void* const* out_z, int prec,
                          int flags);
 
+/* effm_overlap with no device allocation: the split-K partial sums live in
+ * a static device array of 32 MiB (allocated with the library's code object,
+ * held per GPU until the process ends) instead of a context buffer, so the
+ * call makes no hipMalloc / hipFree and allocates nothing.  Results equal
+ * effm_overlap's bit for bit (same split count, same kernels).  Sizes whose
+ * partial sums exceed 32 MiB (an m x n block of one split: m*n*16 bytes for
+ * FP64, m*n*8 for MIXED) return EFFM_ERR_INVALID.  Every call on a GPU
+ * shares that GPU's array: calls that overlap in time on one GPU (other
+ * streams, other threads, other contexts) race.  Device pointers only.
+ * No counterpart in the reference: for callers that cannot allocate
+ * (code under hipGraph capture, tight serving loops). */
+int effm_overlap_noalloc(effm_ctx* ctx, int m, int n, int k, const void* A, int64_t lda,
+                         const void* B, int64_t ldb, void* C, int64_t ldc, int prec, int flags);
+
 /* z <- a z + b y + c x over n complex elements (device pointers, element
  * type from prec, arithmetic in FP64) on the context's stream: the
  * three-term recurrence of a Chebyshev filter in one pass.  No reference
