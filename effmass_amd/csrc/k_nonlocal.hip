@@ -19,6 +19,7 @@
 // factor (src/H_psi/structure_factor.cppm:92-159).
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "effm_internal.hpp"
 
@@ -302,44 +303,70 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
     // row stride of 144 bytes (36 banks): the 16 lanes of one 16-byte LDS
     // read cycle hit 16 distinct 4-bank groups
     constexpr int LDK = BK + 16 / (int)sizeof(C);
-    constexpr int EPT = 64 * BK / 256;                // elements per thread, matrix and slab
     constexpr int KPL = BK / 4;                       // consecutive k per lane and slab
+    // each global load is 16 bytes: one complex128, or two consecutive
+    // complex64 of one row (kchunk is a multiple of 16, so a pair never
+    // straddles a split; only k = K - 1 at the end of the last split can
+    // stand alone)
+    constexpr int VEC = 16 / (int)sizeof(C);
+    constexpr int LPR = BK / VEC;                     // loading lanes per slab row
+    constexpr int RPI = 256 / LPR;                    // rows per load instruction
+    constexpr int NLR = 64 / RPI;                     // loads per lane, matrix and slab
+    using LV = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
     __shared__ __align__(16) C sA[2][64 * LDK];
     __shared__ __align__(16) C sB[2][64 * LDK];
     // Rows / columns past M / N are clamped onto the last one: they only feed
     // outputs that are never stored, so the loads need no branch.
-    const C* ga[EPT];
-    const C* gb[EPT];
-    int lofs[EPT], kofs[EPT];
+    const C* ga[NLR];
+    const C* gb[NLR];
+    int lofs[NLR];
+    const int kofs = (threadIdx.x % LPR) * VEC;
 #pragma unroll
-    for (int q = 0; q < EPT; ++q) {
-        const int e = threadIdx.x + 256 * q;
-        const int row = e / BK, col = e - row * BK;
+    for (int q = 0; q < NLR; ++q) {
+        const int row = (int)threadIdx.x / LPR + q * RPI;
         const int gr = min(tm0 + row, M - 1);
         ga[q] = (sg.A2 != nullptr && gr >= sg.M1) ? sg.A2 + (int64_t)(gr - sg.M1) * lda : A + (int64_t)gr * lda;
         gb[q] = B + (int64_t)min(tn0 + row, N - 1) * ldb;
-        lofs[q] = row * LDK + col;
-        kofs[q] = col;
+        lofs[q] = row * LDK + kofs;
     }
-    C pa[EPT], pb[EPT];
-    // k past the split end reads a clamped address and is zeroed by a select
+    LV pa[NLR], pb[NLR];
+    // k past the split end loads as zero
     auto gload = [&](int kb) {
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int k = kb + kofs[q];
+        const int k = kb + kofs;
+        if constexpr (VEC == 1) {
+            // clamped address, zeroed operands: no branch (and no scratch
+            // array, which an unselected B operand made the compiler use)
             const bool kv = k < ke;
             const int kc = kv ? k : ke - 1;
-            const C x = ga[q][kc];
-            const C y = gb[q][kc];
-            pa[q] = kv ? x : C{0, 0};
-            pb[q] = y;
+#pragma unroll
+            for (int q = 0; q < NLR; ++q) {
+                const C x = ga[q][kc];
+                pa[q] = kv ? x : C{0, 0};
+                const C y = gb[q][kc];
+                pb[q] = kv ? y : C{0, 0};
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NLR; ++q) {
+                LV x{}, y{};
+                if (k + VEC <= ke) {
+                    x = *reinterpret_cast<const LV*>(ga[q] + k);
+                    y = *reinterpret_cast<const LV*>(gb[q] + k);
+                } else if (k < ke) {
+                    const C u = ga[q][k], v = gb[q][k];
+                    x.x = u.x; x.y = u.y;
+                    y.x = v.x; y.y = v.y;
+                }
+                pa[q] = x;
+                pb[q] = y;
+            }
         }
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            sA[buf][lofs[q]] = pa[q];
-            sB[buf][lofs[q]] = pb[q];
+        for (int q = 0; q < NLR; ++q) {
+            *reinterpret_cast<LV*>(&sA[buf][lofs[q]]) = pa[q];
+            *reinterpret_cast<LV*>(&sB[buf][lofs[q]]) = pb[q];
         }
     };
     // lane (i, kk) takes k = kk*KPL + s of rows / columns i of its blocks
@@ -473,8 +500,6 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     const C* __restrict__ B = sg.B;
     const C* __restrict__ A2 = sg.A2;
     const C* __restrict__ B2 = sg.B2;
-    // TWO: the second product continues the k loop, k in [K, 2K) reads A2 / B2
-    const int Ktot = TWO ? 2 * K : K;
     C* __restrict__ out = sg.out;
     const double* __restrict__ diag = sg.diag;
     const C* __restrict__ src = sg.src;
@@ -493,49 +518,75 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
             k3[r][q] = V{0, 0, 0, 0};
         }
     // Slabs of BK k-rows of A (64 consecutive m) and of B (64 consecutive n)
-    // are staged through LDS, double-buffered; each global load instruction
-    // reads one 64-element row segment (whole cache lines).
-    // FP64: 12 k-rows per slab and unpadded rows, 48 KB, so three
-    // workgroups share a CU (measured on the 64-atom apply: 8 rows 680 us,
-    // 16 rows 640 us at two workgroups per CU, 12 rows 590 us); FP32: 16
-    // rows (wider slabs halve the workgroups per CU, narrower add barriers)
+    // are staged through LDS, double-buffered.  Every global load is 16
+    // bytes (one complex128, or two consecutive complex64), consecutive
+    // lanes on consecutive addresses, so one instruction covers whole cache
+    // lines; the row pointers advance by BK rows per slab (no per-element
+    // 64-bit index arithmetic).  FP64: 12 k-rows per slab and unpadded rows,
+    // 48 KB, so three workgroups share a CU (measured on the 64-atom apply:
+    // 8 rows 680 us, 16 rows 640 us at two workgroups per CU, 12 rows
+    // 590 us); FP32: 16 rows (wider slabs halve the workgroups per CU,
+    // narrower add barriers)
     constexpr int BK = sizeof(T) == 8 ? 12 : 16;
     // row stride: a 16-byte read cycle (8 lanes) covers 128 contiguous bytes
     // of one k-row, so FP64 needs no padding; FP32 rows are 64 elements +
     // 128 bytes, consecutive k-rows start 32 banks apart and the two 16-lane
     // halves of an 8-byte read cycle do not collide
     constexpr int LDM = sizeof(T) == 8 ? 64 : 64 + 128 / (int)sizeof(C);
-    constexpr int EPT = 64 * BK / 256;                // elements per thread, matrix and slab
+    constexpr int VEC = 16 / (int)sizeof(C);          // complex per 16-byte load
+    constexpr int CPR = 64 / VEC;                     // loading lanes per slab row
+    constexpr int RSTEP = 256 / CPR;                  // slab rows between a lane's loads
+    constexpr int NL = BK / RSTEP;                    // loads per lane, matrix and slab
+    static_assert(NL * RSTEP == BK, "slab rows must split evenly over the loads");
+    using LV = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
     __shared__ __align__(16) C sA[2][BK * LDM];
     __shared__ __align__(16) C sB[2][BK * LDM];
-    // columns past M / N are clamped onto the last one: they only feed
-    // outputs that are never stored
-    const int lrow = threadIdx.x >> 6, lcol = threadIdx.x & 63;
-    const int mcl = min(tm0 + lcol, M - 1), ncl = min(tn0 + lcol, N - 1);
-    C pa[EPT], pb[EPT];
-    // k past the end reads a clamped address and is zeroed by a select;
-    // TWO: k in [K, 2K) reads row k - K of A2 / B2
-    auto gload = [&](int kb) {
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int k = kb + lrow + 4 * q;
-            const bool kv = k < Ktot;
-            const int kt = kv ? k : Ktot - 1;
-            const bool sec = TWO && kt >= K;
-            const int kc = sec ? kt - K : kt;
-            const C* Ab = sec ? A2 : A;
-            const C* Bb = sec ? B2 : B;
-            const C x = Ab[(int64_t)kc * lda + mcl];
-            const C y = Bb[(int64_t)kc * ldb + ncl];
-            pa[q] = kv ? x : C{0, 0};
-            pb[q] = y;
+    const int lrow = threadIdx.x / CPR, lcol = (threadIdx.x % CPR) * VEC;
+    const int ca = tm0 + lcol, cb = tn0 + lcol;       // first column this lane loads
+    // the k loop runs over nsl slabs of (A, B) and, with TWO, nsl more of
+    // (A2, B2); rows past K of a slab load as zero, columns past M / N load
+    // as zero (they only feed outputs that are never stored)
+    const int nsl = (K + BK - 1) / BK;
+    const int ntot = TWO ? 2 * nsl : nsl;
+    const C* pa = K > 0 ? A + (int64_t)lrow * lda + ca : nullptr;
+    const C* pb = K > 0 ? B + (int64_t)lrow * ldb + cb : nullptr;
+    int krow = 0, lt = 0;                             // next slab: its first row, its index
+    LV ra[NL], rb[NL];
+    auto load16 = [](const C* p, int col, int ncol) -> LV {
+        LV v{};
+        if (col + VEC <= ncol) {
+            v = *reinterpret_cast<const LV*>(p);
+        } else if constexpr (VEC == 2) {
+            if (col < ncol) {                         // the last odd column: one complex
+                const C x = *p;
+                v.x = x.x;
+                v.y = x.y;
+            }
         }
+        return v;
+    };
+    auto gload = [&]() {
+        if (TWO && lt == nsl) {
+            pa = A2 + (int64_t)lrow * lda + ca;
+            pb = B2 + (int64_t)lrow * ldb + cb;
+            krow = 0;
+        }
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const bool kv = krow + lrow + q * RSTEP < K;
+            ra[q] = kv ? load16(pa + (int64_t)(q * RSTEP) * lda, ca, M) : LV{};
+            rb[q] = kv ? load16(pb + (int64_t)(q * RSTEP) * ldb, cb, N) : LV{};
+        }
+        pa += (int64_t)BK * lda;
+        pb += (int64_t)BK * ldb;
+        krow += BK;
+        ++lt;
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            sA[buf][(lrow + 4 * q) * LDM + lcol] = pa[q];
-            sB[buf][(lrow + 4 * q) * LDM + lcol] = pb[q];
+        for (int q = 0; q < NL; ++q) {
+            *reinterpret_cast<LV*>(&sA[buf][(lrow + q * RSTEP) * LDM + lcol]) = ra[q];
+            *reinterpret_cast<LV*>(&sB[buf][(lrow + q * RSTEP) * LDM + lcol]) = rb[q];
         }
     };
     // MFMA k-step s of the slab: lane (i, kk) takes k-row 4 s + kk
@@ -570,14 +621,14 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
                 for (int q = 0; q < 2; ++q) k3[r][q] = mfma16(a[r].y, sc[q], k3[r][q]);
         }
     };
-    if (Ktot > 0) {
-        gload(0);
+    if (K > 0) {
+        gload();
         lstore(0);
         __syncthreads();
         int buf = 0;
-        for (int kb = 0; kb < Ktot; kb += BK) {
-            const bool more = kb + BK < Ktot;
-            if (more) gload(kb + BK);          // next slab in flight during the MFMAs
+        for (int t = 0; t < ntot; ++t) {
+            const bool more = t + 1 < ntot;
+            if (more) gload();                 // next slab in flight during the MFMAs
             mma(buf);
             if (more) lstore(buf ^ 1);
             __syncthreads();
@@ -588,6 +639,44 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
         for (int q = 0; q < 2; ++q) {
             const int n = n0 + q * 16 + i;
             if (n >= N) continue;
+            if constexpr (sizeof(T) == 4) {
+                // FP32 lanes hold four consecutive m: two 16-byte accesses
+                // per operand instead of four 8-byte ones (same arithmetic)
+                const int mb = m0 + r * 16 + kk * 4;
+                if (mb + 3 < M) {
+                    float4 o4[2], s4[2] = {};
+                    double dg[4] = {};
+                    float4* op = reinterpret_cast<float4*>(out + (int64_t)n * ldo + mb);
+                    if (accumulate) o4[0] = op[0], o4[1] = op[1];
+                    if (diag) {
+                        const float4* sp = reinterpret_cast<const float4*>(src + (int64_t)n * lds + mb);
+                        s4[0] = sp[0];
+                        s4[1] = sp[1];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) dg[v] = diag[mb + v];
+                    }
+                    const float* sv = reinterpret_cast<const float*>(s4);
+                    float* ov = reinterpret_cast<float*>(o4);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        double re = (double)(k1[r][q][v] - k3[r][q][v]);
+                        double im = (double)(k1[r][q][v] + k2[r][q][v]);
+                        if (diag) {
+                            re += dg[v] * (double)sv[2 * v];
+                            im += dg[v] * (double)sv[2 * v + 1];
+                        }
+                        if (accumulate) {
+                            re += (double)ov[2 * v];
+                            im += (double)ov[2 * v + 1];
+                        }
+                        ov[2 * v] = (float)re;
+                        ov[2 * v + 1] = (float)im;
+                    }
+                    op[0] = o4[0];
+                    op[1] = o4[1];
+                    continue;
+                }
+            }
             for (int v = 0; v < 4; ++v) {
                 const int m = m0 + r * 16 + out_row<T>(kk, v);
                 if (m >= M) continue;
