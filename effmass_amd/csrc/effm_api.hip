@@ -133,7 +133,7 @@ static int upload(Ctx& c, DevBuf& b, const std::vector<V>& h, const char* what) 
 // LDS), radices from the in-register DFTs of fft_lds.hpp; ties go to the
 // sequence whose largest radix is smallest (fewer registers).
 static bool factor_axis(int n, FftAxis& ax) {
-    static const int radices[] = {12, 10, 9, 8, 7, 6, 5, 4, 3, 2};
+    static const int radices[] = {13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2};
     ax.n = n;
     ax.ns = 0;
     if (n == 1) return true;
@@ -231,7 +231,7 @@ static void timer_resolve(Ctx& c) {
 
 static void release_kpt(Ctx& c, KptDev& k) {
     for (DevBuf* b : {&k.K, &k.kin, &k.gidx, &k.plane_x, &k.x_to_plane, &k.plane_ptr, &k.ent_ig,
-                      &k.ent_pos, &k.row_ptr, &k.rows, &k.P64, &k.P32, &k.dP64, &k.dP32})
+                      &k.ent_pos, &k.row_ptr, &k.rows, &k.ent_rowptr, &k.P64, &k.P32, &k.dP64, &k.dP32})
         dfree(c, *b);
     k = KptDev{};
 }
@@ -619,10 +619,9 @@ int effm_cell_upload(effm_ctx* h, const effm_cell* cell) {
         return fail(c, kErrRuntime, "H|psi> requires ATOM (species / tau missing)");
     FftAxis a1, a2, a3;
     if (!factor_axis(cell->n1, a1) || !factor_axis(cell->n2, a2) || !factor_axis(cell->n3, a3))
-        return fail(c, kErrInvalid, "FFT grid sizes must factor into 2, 3, 5 and 7");
-    size_t plane64 = sizeof(double2) * ((size_t)cell->n2 * cell->n3 + cell->n2 + cell->n3);
-    if (plane64 > 160 * 1024)
-        return fail(c, kErrInvalid, "n2*n3 plane exceeds the 160 KiB LDS of a gfx950 CU");
+        return fail(c, kErrInvalid,
+                    "FFT grid sizes must factor into 2, 3, 5, 7, 11 and 13 (at most " +
+                        std::to_string(kMaxStages) + " factors per axis)");
     release_cell(c);
     c.n1 = cell->n1; c.n2 = cell->n2; c.n3 = cell->n3;
     std::memcpy(c.A, cell->lattice, sizeof(c.A));
@@ -811,6 +810,28 @@ int effm_kpt_upload(effm_ctx* h, int ikpt, int npw, const double* K, const doubl
         ent_ig[e] = g;
         ent_pos[e] = pos[g];
     }
+    // planes too large for the LDS are transformed a chunk of rows at a
+    // time: entries sorted by position within each plane, and the first
+    // entry of every row (the order of a plane's entries does not change any
+    // result: scatter positions and gather targets are distinct)
+    std::vector<int> ent_rowptr;
+    if (plane_lds_bytes<double>(n2, n3) > kPlaneLdsCap) {
+        ent_rowptr.assign((size_t)np * (n2 + 1), 0);
+        std::vector<std::pair<int, int>> pe;
+        for (int p = 0; p < np; ++p) {
+            pe.clear();
+            for (int e = ptr[p]; e < ptr[p + 1]; ++e) pe.emplace_back(ent_pos[e], ent_ig[e]);
+            std::sort(pe.begin(), pe.end());
+            int* rp = ent_rowptr.data() + (size_t)p * (n2 + 1);
+            for (size_t q = 0; q < pe.size(); ++q) {
+                ent_pos[ptr[p] + q] = pe[q].first;
+                ent_ig[ptr[p] + q] = pe[q].second;
+                rp[pe[q].first / n3 + 1]++;
+            }
+            rp[0] = ptr[p];
+            for (int r = 0; r < n2; ++r) rp[r + 1] += rp[r];
+        }
+    }
     // per plane: the (digit-reversed) rows that hold sphere points
     std::vector<int> row_ptr(np + 1, 0), rows;
     {
@@ -844,6 +865,7 @@ int effm_kpt_upload(effm_ctx* h, int ikpt, int npw, const double* K, const doubl
     if ((rc = upload(c, k.ent_pos, ent_pos, "plane positions"))) return rc;
     if ((rc = upload(c, k.row_ptr, row_ptr, "plane row ptr"))) return rc;
     if ((rc = upload(c, k.rows, rows, "plane rows"))) return rc;
+    if ((rc = upload(c, k.ent_rowptr, ent_rowptr, "plane row entries"))) return rc;
     k.nplanes = np;
     int mx = 0;
     for (int p = 0; p < np; ++p) mx = std::max(mx, ptr[p + 1] - ptr[p]);

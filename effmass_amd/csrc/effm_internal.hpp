@@ -85,6 +85,10 @@ struct KptDev {
     DevBuf row_ptr;      // int[nplanes+1]: per plane, its rows holding sphere points
     DevBuf rows;         //   (digit-reversed row index j', ascending)
     int max_plane_entries = 0;
+    // grids whose n2 x n3 FP64 plane exceeds the LDS: entries sorted by
+    // position within each plane, and per plane the first entry of each
+    // (digit-reversed) row, [nplanes][n2 + 1]
+    DevBuf ent_rowptr;
     // projector matrices, built on first use: P[c][g]; dP[alpha][c][g]
     DevBuf P64, P32, dP64, dP32;
 };
@@ -192,6 +196,16 @@ struct TableDev {
     double norm;     // 1/sqrt(Omega)
     double dbeta0;   // normalised dbeta/dq at q = 0
 };
+
+// Largest dynamic LDS of one workgroup on gfx950; planes (plus twiddles)
+// beyond it take the split-plane kernels.
+constexpr size_t kPlaneLdsCap = 160 * 1024;
+
+// LDS bytes of one whole-plane workgroup (plane + both axes' twiddles).
+template <typename T>
+inline size_t plane_lds_bytes(int n2, int n3) {
+    return sizeof(typename Cx<T>::type) * ((size_t)n2 * n3 + n2 + n3);
+}
 
 // Row pitch (complex elements) of the FFT plane workspace.  FP32 rows are
 // padded to a 128-byte line (the x-pencil pass then reads whole lines);

@@ -182,6 +182,52 @@ __device__ __forceinline__ void dft7(cx_t<T>* y) {
     y[4] = csub(m3, r3);
 }
 
+// Any odd prime P (11, 13 in the plans) by the symmetric pairs
+// t_u = y_u + y_{P-u}, d_u = y_u - y_{P-u}, u = 1 .. (P-1)/2:
+//   y_q, y_{P-q} = y_0 + sum_u cos(2 pi u q / P) t_u
+//                  +- S i sum_u sin(2 pi u q / P) d_u.
+// The recursion over (q, u) makes every cosine and sine a compile-time
+// constant.
+template <int P, int S, typename T, int Q, int U>
+struct OddPairs {
+    static constexpr int H = (P - 1) / 2;
+    static __device__ __forceinline__ void sums(cx_t<T>& m, cx_t<T>& r, const cx_t<T>* t, const cx_t<T>* d) {
+        constexpr T cq = (T)ce::cos2pi(U * Q, P);
+        constexpr T sq = (T)ce::sin2pi(U * Q, P);
+        m.x += cq * t[U - 1].x;
+        m.y += cq * t[U - 1].y;
+        r.x += sq * d[U - 1].x;
+        r.y += sq * d[U - 1].y;
+        if constexpr (U < H) OddPairs<P, S, T, Q, U + 1>::sums(m, r, t, d);
+    }
+    static __device__ __forceinline__ void outputs(cx_t<T>* y, const cx_t<T>* t, const cx_t<T>* d) {
+        using C = cx_t<T>;
+        C m = y[0], r = C{0, 0};
+        OddPairs<P, S, T, Q, 1>::sums(m, r, t, d);
+        r = cmul_si<S>(r);
+        y[Q] = cadd(m, r);
+        y[P - Q] = csub(m, r);
+        if constexpr (Q < H) OddPairs<P, S, T, Q + 1, 1>::outputs(y, t, d);
+    }
+};
+
+template <int P, int S, typename T>
+__device__ __forceinline__ void dft_odd(cx_t<T>* y) {
+    using C = cx_t<T>;
+    constexpr int H = (P - 1) / 2;
+    C t[H], d[H];
+#pragma unroll
+    for (int u = 1; u <= H; ++u) {
+        t[u - 1] = cadd(y[u], y[P - u]);
+        d[u - 1] = csub(y[u], y[P - u]);
+    }
+    C sum = y[0];
+#pragma unroll
+    for (int u = 0; u < H; ++u) sum = cadd(sum, t[u]);
+    OddPairs<P, S, T, 1, 1>::outputs(y, t, d);      // reads y[0] before it is replaced
+    y[0] = sum;
+}
+
 // Composite P = P1 * P2 by one Cooley-Tukey split in registers:
 // u = P2 u1 + u2, q = q1 + P1 q2:
 //   Z[u2][q1] = DFT_P1 over u1 of y[P2 u1 + u2], times w^(u2 q1),
@@ -272,6 +318,7 @@ __device__ __forceinline__ void dft(cx_t<T>* y) {
     else if constexpr (P == 15) dft_pfa<3, 5, S, T>(y);
     else if constexpr (P == 16) dft_split<4, 4, S, T>(y);
     else if constexpr (P == 7) dft7<S, T>(y);
+    else if constexpr (P == 11 || P == 13) dft_odd<P, S, T>(y);
 }
 
 template <int P, int S, bool DIT, typename T>
@@ -324,7 +371,8 @@ __device__ void fft_stage(cx_t<T>* buf, int n, int p, int L, int es, int ls, int
         switch (p) {
 #define EFFM_RADIX(P) case P: butterfly<P, S, DIT, T>(buf, base, ues, k1, tstep, tw); break;
             EFFM_RADIX(2) EFFM_RADIX(3) EFFM_RADIX(4) EFFM_RADIX(5) EFFM_RADIX(6)
-            EFFM_RADIX(7) EFFM_RADIX(8) EFFM_RADIX(9) EFFM_RADIX(10) EFFM_RADIX(12)
+            EFFM_RADIX(7) EFFM_RADIX(8) EFFM_RADIX(9) EFFM_RADIX(10) EFFM_RADIX(11)
+            EFFM_RADIX(12) EFFM_RADIX(13)
 #undef EFFM_RADIX
             default: break;
         }

@@ -44,6 +44,7 @@ struct LocalSeg {
     const int* row_ptr;       // [nplanes + 1]
     const int* rows;          // rows of each plane that hold sphere points
     const double* kin;        // [npw]
+    const int* ent_rowptr;    // [nplanes][n2 + 1] (split-plane grids) or NULL
     cx_t<T>* work;            // [nb][nplanes][ws_plane] or [nb][n2][nplanes][pitch]
     // element (band b, plane p, row j, z) of the compile-time kernels sits
     // at work[b nplanes ws_plane + p wsp + j wsr + z]: plane-major
@@ -197,6 +198,135 @@ k_plane_r2g(const LocalBatch<T> bt, T scale, int accumulate, FftAxis a2, FftAxis
         }
         hs[ig] = C{(T)re, (T)im};
     }
+}
+
+// ---------------------------------------------------------------------------
+//  Split-plane kernels: grids whose n2 x n3 plane does not fit the LDS
+//  (e.g. 108^3, 120^3, 128^3 in FP64).  The 2D plane transform becomes two
+//  passes through the workspace (plane-major, rows in digit-reversed order
+//  between the passes), each workgroup holding a chunk of rows or columns:
+//    G2R  rows:  zero R rows, scatter their sphere entries (sorted by row,
+//                ent_rowptr), DIT backward FFT along k, store the rows;
+//         cols:  load C columns, DIT backward FFT along j, store natural;
+//    R2G  cols:  load C natural columns, DIF forward FFT along j, store;
+//         rows:  load R rows, DIF forward FFT along k, gather
+//                hpsi = [hpsi +] kin psi + V psi / N.
+//  Items are flat over (segment, plane, band, chunk), chunk fastest.
+// ---------------------------------------------------------------------------
+struct SplitShape {
+    int rows;      // rows per row-pass workgroup
+    int nrc;       // row chunks per plane
+    int cols;      // columns per column-pass workgroup
+    int ncc;       // column chunks per plane
+};
+
+template <typename T, bool G2R>
+__global__ void __launch_bounds__(256)
+k_split_rows(const LocalBatch<T> bt, SplitShape sh, int nitems, T scale, int accumulate, FftAxis a2, FftAxis a3,
+             const cx_t<T>* __restrict__ tw3) {
+    using C = cx_t<T>;
+    const int item = blockIdx.x;
+    if (item >= nitems) return;
+    const LocalSeg<T>& sg = bt.s[seg_of(bt, item)];
+    const int local = item - sg.item0;
+    const int rc = local % sh.nrc;
+    const int p = (local / sh.nrc) % sg.nplanes;
+    const int b = local / sh.nrc / sg.nplanes;
+    const int n2 = a2.n, n3 = a3.n;
+    const int r0 = rc * sh.rows, nr = min(sh.rows, n2 - r0);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* buf = reinterpret_cast<C*>(smem_raw);          // [nr][n3]
+    C* t3 = buf + (size_t)sh.rows * n3;
+    const int wp = ws_pitch<T>(n3);
+    C* ws = sg.work + ((int64_t)b * sg.nplanes + p) * ws_plane<T>(n2, n3) + (int64_t)r0 * wp;
+    for (int i = threadIdx.x; i < n3; i += blockDim.x) t3[i] = tw3[i];
+    const int* rp = sg.ent_rowptr + (size_t)p * (n2 + 1);
+    const int e0 = rp[r0], e1 = rp[r0 + nr];
+    const int base = r0 * n3;
+    if (G2R) {
+        for (int i = threadIdx.x; i < nr * n3; i += blockDim.x) buf[i] = C{0, 0};
+        __syncthreads();
+        const C* ps = sg.psi + (int64_t)b * sg.ld;
+        for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) buf[sg.ent_pos[e] - base] = ps[sg.ent_ig[e]];
+        __syncthreads();
+        fft_lds<+1, true, T>(buf, a3, 1, n3, nr, t3);
+        for (int i = threadIdx.x; i < nr * n3; i += blockDim.x) {
+            const int r = i / n3;
+            ws[(int64_t)r * wp + (i - r * n3)] = buf[i];
+        }
+    } else {
+        for (int i = threadIdx.x; i < nr * n3; i += blockDim.x) {
+            const int r = i / n3;
+            buf[i] = ws[(int64_t)r * wp + (i - r * n3)];
+        }
+        __syncthreads();
+        fft_lds<-1, false, T>(buf, a3, 1, n3, nr, t3);
+        const C* ps = sg.psi + (int64_t)b * sg.ld;
+        C* hs = sg.hpsi + (int64_t)b * sg.ldo;
+        for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+            const int ig = sg.ent_ig[e];
+            const C v = buf[sg.ent_pos[e] - base];
+            const C sv = ps[ig];
+            const double kn = sg.kin[ig];
+            // kinetic term in FP64 (callable.cpp:119-126), then the gathered
+            // local-potential term (callable.cpp:171-182)
+            double re = kn * (double)sv.x + (double)(v.x * scale);
+            double im = kn * (double)sv.y + (double)(v.y * scale);
+            if (accumulate) {
+                const C h = hs[ig];
+                re += (double)h.x;
+                im += (double)h.y;
+            }
+            hs[ig] = C{(T)re, (T)im};
+        }
+    }
+}
+
+// Column pass: DIT backward (S = +1, digit-reversed rows in, natural out)
+// or DIF forward (S = -1, natural in, digit-reversed rows out), in place.
+template <typename T, int S>
+__global__ void __launch_bounds__(256)
+k_split_cols(const LocalBatch<T> bt, SplitShape sh, int nitems, FftAxis a2, int n3, const cx_t<T>* __restrict__ tw2) {
+    using C = cx_t<T>;
+    const int item = blockIdx.x;
+    if (item >= nitems) return;
+    const LocalSeg<T>& sg = bt.s[seg_of(bt, item)];
+    const int local = item - sg.item0;
+    const int cc = local % sh.ncc;
+    const int p = (local / sh.ncc) % sg.nplanes;
+    const int b = local / sh.ncc / sg.nplanes;
+    const int n2 = a2.n;
+    const int c0 = cc * sh.cols, nc = min(sh.cols, n3 - c0);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* buf = reinterpret_cast<C*>(smem_raw);          // [n2][cols], columns adjacent
+    C* t2 = buf + (size_t)n2 * sh.cols;
+    const int wp = ws_pitch<T>(n3);
+    C* ws = sg.work + ((int64_t)b * sg.nplanes + p) * ws_plane<T>(n2, n3) + c0;
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) t2[i] = tw2[i];
+    for (int i = threadIdx.x; i < n2 * nc; i += blockDim.x) {
+        const int j = i / nc, z = i - j * nc;
+        buf[j * sh.cols + z] = ws[(int64_t)j * wp + z];
+    }
+    __syncthreads();
+    fft_lds<S, (S > 0), T>(buf, a2, sh.cols, 1, nc, t2);
+    for (int i = threadIdx.x; i < n2 * nc; i += blockDim.x) {
+        const int j = i / nc, z = i - j * nc;
+        ws[(int64_t)j * wp + z] = buf[j * sh.cols + z];
+    }
+}
+
+// Chunk sizes of the split-plane kernels: about 64 KB of LDS per workgroup
+// (two per CU), at least one row / column.
+template <typename T>
+static SplitShape split_shape(int n2, int n3) {
+    constexpr size_t kChunkLds = 64 * 1024;
+    const size_t S = sizeof(cx_t<T>);
+    SplitShape sh{};
+    sh.rows = (int)std::max<size_t>(1, std::min<size_t>((size_t)n2, kChunkLds / (S * n3) - 1));
+    sh.nrc = (n2 + sh.rows - 1) / sh.rows;
+    sh.cols = (int)std::max<size_t>(1, std::min<size_t>((size_t)n3, kChunkLds / (S * n2) - 1));
+    sh.ncc = (n3 + sh.cols - 1) / sh.cols;
+    return sh;
 }
 
 // ---------------------------------------------------------------------------
@@ -765,7 +895,7 @@ static void pencil_ct(Ctx& c, int n, const LocalBatch<T>& bt, const T* vl, const
 
 template <typename T>
 static size_t plane_lds_bytes(const Ctx& c) {
-    return sizeof(cx_t<T>) * ((size_t)c.n2 * c.n3 + c.n2 + c.n3);
+    return plane_lds_bytes<T>(c.n2, c.n3);
 }
 
 template <typename T>
@@ -779,6 +909,58 @@ size_t local_work_bytes(const Ctx& c, const ApplySeg* segs, int nseg, int nb) {
 // Workspace of the compile-time kernels row-major ([band][row][plane][z])
 // rather than plane-major ([band][plane][row][z]).
 constexpr bool kRowMajorWork = true;
+
+// Split-plane sequence (planes beyond the LDS): rows, columns, x pencils,
+// columns, rows + gather, all on plane-major workspace.
+template <typename T>
+static hipError_t launch_local_split(Ctx& c, LocalBatch<T> bt, int nb, bool accumulate, const cx_t<T>* tw1,
+                                     const cx_t<T>* tw2, const cx_t<T>* tw3, const T* vl) {
+    using C = cx_t<T>;
+    constexpr int TZ = 16;
+    const SplitShape sh = split_shape<T>(c.n2, c.n3);
+    LocalBatch<T> br = bt, bcl = bt;
+    int nr_items = 0, nc_items = 0;
+    for (int i = 0; i < bt.nseg; ++i) {
+        br.s[i].item0 = nr_items;
+        bcl.s[i].item0 = nc_items;
+        nr_items += bt.s[i].nplanes * nb * sh.nrc;
+        nc_items += bt.s[i].nplanes * nb * sh.ncc;
+    }
+    const size_t lds_rows = sizeof(C) * ((size_t)sh.rows * c.n3 + c.n3);
+    const size_t lds_cols = sizeof(C) * ((size_t)c.n2 * sh.cols + c.n2);
+    const size_t lds_pencil = sizeof(C) * ((size_t)c.n1 * TZ + c.n1);
+    hipError_t e;
+    const void* fns[] = {reinterpret_cast<const void*>(&k_split_rows<T, true>),
+                         reinterpret_cast<const void*>(&k_split_rows<T, false>),
+                         reinterpret_cast<const void*>(&k_split_cols<T, +1>),
+                         reinterpret_cast<const void*>(&k_split_cols<T, -1>),
+                         reinterpret_cast<const void*>(&k_xpencil_vloc<T, TZ>)};
+    const size_t ldss[] = {lds_rows, lds_rows, lds_cols, lds_cols, lds_pencil};
+    for (int f = 0; f < 5; ++f)
+        if ((e = hipFuncSetAttribute(fns[f], hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldss[f])) != hipSuccess)
+            return e;
+    const T scale = (T)(1.0 / ((double)c.n1 * c.n2 * c.n3));
+    timer_begin(c);
+    hipLaunchKernelGGL((k_split_rows<T, true>), dim3(nr_items), dim3(256), lds_rows, c.stream, br, sh, nr_items,
+                       scale, 0, c.ax2, c.ax3, tw3);
+    hipLaunchKernelGGL((k_split_cols<T, +1>), dim3(nc_items), dim3(256), lds_cols, c.stream, bcl, sh, nc_items, c.ax2,
+                       c.n3, tw2);
+    timer_end(c, "plane_g2r");
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    timer_begin(c);
+    const int nzc = (c.n3 + TZ - 1) / TZ;
+    hipLaunchKernelGGL((k_xpencil_vloc<T, TZ>), dim3(c.n2 * nzc, nb, bt.nseg), dim3(256), lds_pencil, c.stream, bt,
+                       static_cast<const int*>(c.perm1.ptr), vl, c.ax1, c.n2, c.n3, tw1);
+    timer_end(c, "xpencil_vloc");
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    timer_begin(c);
+    hipLaunchKernelGGL((k_split_cols<T, -1>), dim3(nc_items), dim3(256), lds_cols, c.stream, bcl, sh, nc_items, c.ax2,
+                       c.n3, tw2);
+    hipLaunchKernelGGL((k_split_rows<T, false>), dim3(nr_items), dim3(256), lds_rows, c.stream, br, sh, nr_items,
+                       scale, accumulate ? 1 : 0, c.ax2, c.ax3, tw3);
+    timer_end(c, "plane_r2g");
+    return hipGetLastError();
+}
 
 template <typename T>
 hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, bool accumulate,
@@ -816,6 +998,7 @@ hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, 
         s.row_ptr = static_cast<const int*>(k.row_ptr.ptr);
         s.rows = static_cast<const int*>(k.rows.ptr);
         s.kin = static_cast<const double*>(k.kin.ptr);
+        s.ent_rowptr = static_cast<const int*>(k.ent_rowptr.ptr);
         s.work = wcur;
         s.nplanes = k.nplanes;
         s.wsp = rowmaj ? (int64_t)ws_pitch<T>(c.n3) : ws_plane<T>(c.n2, c.n3);
@@ -835,6 +1018,11 @@ hipError_t launch_local(Ctx& c, const ApplySeg* segs, int nseg, int b0, int nb, 
     const size_t lds_pencil = sizeof(C) * ((size_t)c.n1 * TZ + c.n1);
     hipError_t e;
 
+    if (!plct && lds_plane > kPlaneLdsCap) {
+        for (int i = 0; i < nseg; ++i)
+            if (!segs[i].k->ent_rowptr.ptr) return hipErrorInvalidValue;   // built at upload for such grids
+        return launch_local_split<T>(c, bt, nb, accumulate, tw1, tw2, tw3, vl);
+    }
     if (!plct || !pect) {
         // generic kernels: dynamic LDS sized by this cell's grid
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_g2r<T>),

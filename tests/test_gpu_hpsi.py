@@ -293,7 +293,8 @@ def test_multi_k_ragged():
 
 @pytest.mark.parametrize("grid", [(28, 30, 32), (42, 45, 40), (24, 20, 36), (48, 49, 50),
                                   (15, 15, 15), (16, 16, 16), (18, 18, 18), (20, 20, 20), (25, 25, 25),
-                                  (40, 24, 24), (45, 45, 45), (48, 25, 25), (48, 45, 45), (16, 40, 40)])
+                                  (40, 24, 24), (45, 45, 45), (48, 25, 25), (48, 45, 45), (16, 40, 40),
+                                  (22, 26, 33), (26, 22, 39)])
 def test_fft_radix_plans(grid):
     """Grids whose axes use radices 2..12 (incl. 7) in the runtime LDS plan,
     and every compile-time two-stage plan (square planes and pencils of
@@ -301,7 +302,7 @@ def test_fft_radix_plans(grid):
     radices are coprime (15, 20, 40, 45), Cooley-Tukey otherwise, and the
     two mixed between the pencil and the plane axes ((40, 24, 24): prime-
     factor pencil, Cooley-Tukey planes; (48, 45, 45), (16, 40, 40): the
-    reverse)."""
+    reverse), and axes with the prime radices 11 and 13 (22, 26, 33, 39)."""
     wl = synth.custom(*synth.fcc_primitive(10.26), grid, 12.0,
                       [np.array([0.13, 0.0, -0.27])], 4, seed=sum(grid))
     got = run_h(wl)
@@ -428,3 +429,32 @@ def test_ge_d_projector_upf():
         eg = max(band_rel(gr[a], gref[a]) for a in range(3))
         log(f"Ge PD04 {prec}: H {e:.2e}, dH/dk {eg:.2e}")
         assert e <= tol and eg <= tol
+
+
+@pytest.mark.parametrize("grid", [(108, 108, 108), (120, 120, 120), (128, 128, 128), (100, 112, 104)])
+def test_large_grids(grid):
+    """Grids whose FP64 n2 x n3 plane exceeds the 160 KiB LDS take the
+    split-plane kernels (row and column passes through the workspace);
+    their FP32 planes still fit and take the whole-plane runtime-plan
+    kernels.  FP64 H|psi> against the oracle at 1e-12; mixed within 2e-5 and
+    its Rayleigh quotients within 1e-6 Ry of the FP64 path's."""
+    torch = _torch()
+    A, tau = synth.diamond_cubic(10.26)
+    wl = synth.custom(A, tau, grid, 25.0, [np.array([0.1, -0.05, 0.2])], 3, seed=sum(grid))
+    kp = wl.kpts[0]
+    ref = oracle.hpsi(wl.cell, kp, wl.psi[0], nthreads=8)
+    H = Hamiltonian(wl.cell)
+    H.load_kpoint(0, kp)
+    psi = torch.from_numpy(wl.psi[0]).cuda()
+    h = torch.zeros_like(psi)
+    H.at_k(0)(psi, h)
+    p32 = psi.to(torch.complex64)
+    h32 = torch.zeros_like(p32)
+    H.at_k(0)(p32, h32, prec="mixed")
+    torch.cuda.synchronize()
+    e = band_rel(h.cpu().numpy(), ref)
+    e32 = band_rel(h32.cpu().numpy().astype(np.complex128), ref)
+    de = np.abs(H.rayleigh(p32, h32, prec="mixed").cpu().numpy() - H.rayleigh(psi, h).cpu().numpy()).max()
+    log(f"grid {grid} npw {kp.npw}: FP64 {e:.2e}, mixed {e32:.2e}, |dE| {de:.2e} Ha")
+    H.close()
+    assert e <= 1e-12 and e32 <= 2e-5 and de <= RY_TOL_HA
