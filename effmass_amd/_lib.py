@@ -31,7 +31,7 @@ EXPORTS = (
     "effm_mem_info", "effm_cell_upload", "effm_kpt_upload", "effm_kpt_dim",
     "effm_kpt_release", "effm_hpsi", "effm_dhdk", "effm_hpsi_multi", "effm_dhdk_multi",
     "effm_hpsi_dhdk_multi", "effm_lincomb", "effm_rayleigh",
-    "effm_overlap", "effm_overlap_noalloc", "effm_autotune", "effm_set_timing", "effm_get_timing", "effm_reset_timing",
+    "effm_overlap", "effm_overlap_noalloc", "effm_kp_mass", "effm_autotune", "effm_set_timing", "effm_get_timing", "effm_reset_timing",
 )
 
 
@@ -121,6 +121,7 @@ def load() -> C.CDLL:
     lib.effm_autotune.argtypes = [vp, i32, i32, i32, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     lib.effm_overlap.argtypes = [vp, i32, i32, i32, vp, i64, vp, i64, vp, i64, i32, i32]
     lib.effm_overlap_noalloc.argtypes = [vp, i32, i32, i32, vp, i64, vp, i64, vp, i64, i32, i32]
+    lib.effm_kp_mass.argtypes = [vp, i32, vp, vp, i64, i32, vp, vp, C.c_double, vp, i32]
     lib.effm_set_timing.argtypes = [vp, i32]
     lib.effm_get_timing.argtypes = [vp, C.c_char_p, dp, C.POINTER(C.c_int64)]
     lib.effm_reset_timing.argtypes = [vp]

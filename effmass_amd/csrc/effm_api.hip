@@ -1172,6 +1172,32 @@ int effm_rayleigh(effm_ctx* h, int npw, int nb, const void* psi, int64_t ld, con
     return kOk;
 }
 
+int effm_kp_mass(effm_ctx* h, int nb, const double* E, const void* const* V, int64_t ldv, int nsel,
+                 const int* bands, const double* d2h, double degen_tol, double* inv_mass, int flags) {
+    if (!h) return kErrInvalid;
+    Ctx& c = h->c;
+    (void)hipSetDevice(c.device);
+    if (nsel < 0 || nb < 0) return fail(c, kErrInvalid, "negative band count");
+    if (nsel == 0) return kOk;
+    if (nsel > kKpMaxBands)
+        return fail(c, kErrInvalid, "at most " + std::to_string(kKpMaxBands) + " bands per effm_kp_mass call");
+    if (!E || !V || !V[0] || !V[1] || !V[2] || !bands || !d2h || !inv_mass)
+        return fail(c, kErrInvalid, "k.p operands must not be NULL");
+    if (ldv < nb) return fail(c, kErrInvalid, "ldv smaller than nb");
+    if (!(degen_tol >= 0.0)) return fail(c, kErrInvalid, "degeneracy tolerance must be non-negative");
+    if (flags & (EFFM_F_HOST | EFFM_F_GRAPH)) return fail(c, kErrInvalid, "effm_kp_mass takes device pointers only");
+    KpBands bs{};
+    for (int i = 0; i < nsel; ++i) {
+        if (bands[i] < 0 || bands[i] >= nb) return fail(c, kErrOutOfRange, "band index out of range");
+        bs.n[i] = bands[i];
+    }
+    timer_begin(c);
+    hipError_t e = launch_kp_sum(c.stream, bs, nsel, nb, E, V, ldv, d2h, degen_tol, inv_mass);
+    timer_end(c, "kp_sum");
+    if (e != hipSuccess) return hip_fail(c, e, "k.p sum");
+    return kOk;
+}
+
 int effm_set_timing(effm_ctx* h, int enable) {
     if (!h) return kErrInvalid;
     h->c.timing = enable != 0;

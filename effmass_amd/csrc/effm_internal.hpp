@@ -291,6 +291,14 @@ template <typename T>
 hipError_t launch_lincomb(Ctx& c, int64_t n, double a, void* z, double b, const void* y, double cc,
                           const void* x);
 
+// selected bands of one k.p launch (by value)
+constexpr int kKpMaxBands = 64;
+struct KpBands {
+    int n[kKpMaxBands];
+};
+hipError_t launch_kp_sum(hipStream_t s, const KpBands& bs, int nsel, int nb, const double* E,
+                         const void* const* V, int64_t ldv, const double* d2h, double tol, double* out);
+
 template <typename T>
 hipError_t launch_rayleigh(hipStream_t s, int npw, int nb, const void* psi, int64_t ld,
                            const void* hpsi, int64_t ldh, double* E);

@@ -1025,6 +1025,49 @@ template hipError_t launch_rayleigh<double>(hipStream_t, int, int, const void*, 
 template hipError_t launch_rayleigh<float>(hipStream_t, int, int, const void*, int64_t,
                                            const void*, int64_t, double*);
 
+// ---------------------------------------------------------------------------
+//  k.p sum over states in FP64: one workgroup per (selected band n, a, b)
+//    M_ab = (d2h_ab + d2h_ba) / 2
+//         + sum_{m: |E_n - E_m| > tol, m != n} Re(V_a[n][m] V_b[m][n] + V_b[n][m] V_a[m][n]) / (E_n - E_m)
+//  (the symmetrised 2 Re sum of the k.p expression, denominators in FP64)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_kp_sum(const KpBands bs, int nb, const double* __restrict__ E, const double2* __restrict__ V0,
+         const double2* __restrict__ V1, const double2* __restrict__ V2, int64_t ldv,
+         const double* __restrict__ d2h, double tol, double* __restrict__ out) {
+    __shared__ double red[256];
+    const int sel = blockIdx.x, a = blockIdx.y / 3, b = blockIdx.y % 3;
+    const int n = bs.n[sel];
+    const double2* Va = a == 0 ? V0 : (a == 1 ? V1 : V2);
+    const double2* Vb = b == 0 ? V0 : (b == 1 ? V1 : V2);
+    const double en = E[n];
+    double acc = 0.0;
+    for (int m = threadIdx.x; m < nb; m += blockDim.x) {
+        const double d = en - E[m];
+        if (m == n || !(fabs(d) > tol)) continue;
+        const double2 x = Va[(int64_t)n * ldv + m], y = Vb[(int64_t)m * ldv + n];
+        const double2 u = Vb[(int64_t)n * ldv + m], v = Va[(int64_t)m * ldv + n];
+        acc += ((x.x * y.x - x.y * y.y) + (u.x * v.x - u.y * v.y)) / d;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double* dd = d2h + (size_t)sel * 9;
+        out[(size_t)sel * 9 + a * 3 + b] = 0.5 * (dd[a * 3 + b] + dd[b * 3 + a]) + red[0];
+    }
+}
+
+hipError_t launch_kp_sum(hipStream_t s, const KpBands& bs, int nsel, int nb, const double* E,
+                         const void* const* V, int64_t ldv, const double* d2h, double tol, double* out) {
+    hipLaunchKernelGGL(k_kp_sum, dim3(nsel, 9), dim3(256), 0, s, bs, nb, E, static_cast<const double2*>(V[0]),
+                       static_cast<const double2*>(V[1]), static_cast<const double2*>(V[2]), ldv, d2h, tol, out);
+    return hipGetLastError();
+}
+
 // z <- a z + b y + c x elementwise over n complex values, in FP64 for either
 // storage precision: one pass where three scale / add passes would read z
 // three times (the eigensolver's Chebyshev three-term recurrence).

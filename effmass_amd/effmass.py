@@ -14,7 +14,10 @@ difference in k of the analytic <n|dH/dk_a(k +- delta e_b)|n>
 exact for the kinetic part and O(delta^2) for the nonlocal part.
 
 Velocity matrices, the sum over states and the (E_n - E_m) denominators are
-FP64 in both precisions; only the H|psi> / dH/dk applications follow `prec`.
+FP64 in both precisions and stay on the device (effm_overlap for the
+matrices, effm_kp_mass for the sum); only the H|psi> / dH/dk applications
+follow `prec`.  kp_sum below is the same sum on the host, for dense-matrix
+checks on small cells.
 States m within `degen_tol` of E_n are left out of the sum (their
 contribution is undefined for a degenerate level; the result is then the
 mass of the level projected out of its degenerate partners).
@@ -53,7 +56,8 @@ def shifted_kpoint(kp: synth.KPoint, A: np.ndarray, dk_cart) -> synth.KPoint:
 
 def kp_sum(E, V, d2h, n, degen_tol=1e-6):
     """(1/m*)_ab for band n from eigenvalues E [nb], velocity matrices
-    V[a][m, l] = <m|dH/dk_a|l> (3 x nb x nb) and <n|d^2H|n> (3x3)."""
+    V[a][m, l] = <m|dH/dk_a|l> (3 x nb x nb) and <n|d^2H|n> (3x3), on the
+    host (numpy): the dense-matrix form of effm_kp_mass."""
     E = np.asarray(E, dtype=np.float64)
     d = E[n] - E
     keep = np.abs(d) > degen_tol
@@ -67,14 +71,15 @@ def kp_sum(E, V, d2h, n, degen_tol=1e-6):
 
 
 def velocity_matrices(H: Hamiltonian, ikpt: int, X: torch.Tensor, prec="fp64"):
-    """V[a] = <m|dH/dk_a|n> (FP64, nb x nb) for the rows of X."""
+    """V[a] = <m|dH/dk_a|n> (FP64 complex128 device tensors, nb x nb) for
+    the rows of X."""
     Xp = X.to(torch.complex128 if prec == "fp64" else torch.complex64).contiguous()
     D = [torch.zeros_like(Xp) for _ in range(3)]
     H.gradient().at_k(ikpt)(Xp, GradHPsi(*D), prec=prec, overwrite=True)
     X64 = X.to(torch.complex128).contiguous()
     # FP64 split-K products on the engine (a long-K complex128 matmul on
     # the BLAS library runs at a fraction of the MFMA rate)
-    return [H.overlap(X64, d.to(torch.complex128)).cpu().numpy() for d in D]
+    return [H.overlap(X64, d.to(torch.complex128)) for d in D]
 
 
 def d2h_expect(H: Hamiltonian, kp: synth.KPoint, A, X: torch.Tensor, bands, prec="fp64",
@@ -86,7 +91,7 @@ def d2h_expect(H: Hamiltonian, kp: synth.KPoint, A, X: torch.Tensor, bands, prec
         slot = max(H._npw.keys(), default=-1) + 1
     Xb = X[bands].to(torch.complex128 if prec == "fp64" else torch.complex64).contiguous()
     X64 = X[bands].to(torch.complex128)
-    out = np.zeros((len(bands), 3, 3))
+    out = torch.zeros((len(bands), 3, 3), dtype=torch.float64, device=X.device)
     for b in range(3):
         vals = []
         for sgn in (+1.0, -1.0):
@@ -95,22 +100,25 @@ def d2h_expect(H: Hamiltonian, kp: synth.KPoint, A, X: torch.Tensor, bands, prec
             H.load_kpoint(slot, shifted_kpoint(kp, A, dk))
             D = [torch.zeros_like(Xb) for _ in range(3)]
             H.gradient().at_k(slot)(Xb, GradHPsi(*D), prec=prec, overwrite=True)
-            vals.append(np.stack([np.real((X64.conj() * d.to(torch.complex128)).sum(1)).cpu().numpy()
-                                  for d in D], axis=1))      # [nbands, a]
+            vals.append(torch.stack([(X64.conj() * d.to(torch.complex128)).sum(1).real for d in D],
+                                    dim=1))                  # [nbands, a], FP64 on the device
         out[:, :, b] = (vals[0] - vals[1]) / (2.0 * delta)
     H.release_kpoint(slot)
-    return 0.5 * (out + out.transpose(0, 2, 1))
+    return 0.5 * (out + out.transpose(1, 2))
 
 
 def effective_mass(H: Hamiltonian, ikpt: int, kp: synth.KPoint, A, X: torch.Tensor, E, bands,
                    prec="fp64", delta=1e-3, degen_tol=1e-6) -> MassResult:
     """(1/m*) for `bands` from the eigenpairs (X rows, E) of H at ikpt."""
-    E = np.asarray(E.detach().cpu().numpy() if torch.is_tensor(E) else E, dtype=np.float64)
+    Ed = (E if torch.is_tensor(E) else torch.as_tensor(np.asarray(E, dtype=np.float64))).to(
+        device=X.device, dtype=torch.float64)
     V = velocity_matrices(H, ikpt, X, prec)
     d2 = d2h_expect(H, kp, A, X, bands, prec, delta)
-    inv, excl = [], []
-    for i, n in enumerate(bands):
-        m, ex = kp_sum(E, V, d2[i], n, degen_tol)
-        inv.append(m)
-        excl.append(ex)
-    return MassResult(bands=list(bands), inv_mass=np.array(inv), d2h=d2, E=E, excluded=excl)
+    inv = H.kp_mass(Ed, V, bands, d2, degen_tol).cpu().numpy()
+    En = Ed.cpu().numpy()
+    excl = []
+    for n in bands:
+        keep = np.abs(En[n] - En) > degen_tol
+        keep[n] = False
+        excl.append(np.nonzero(~keep)[0].tolist())
+    return MassResult(bands=list(bands), inv_mass=inv, d2h=d2.cpu().numpy(), E=En, excluded=excl)

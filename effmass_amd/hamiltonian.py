@@ -317,6 +317,32 @@ class Hamiltonian:
                                           Cm.data_ptr(), max(n, 1), p, 0), self._ctx, "effm_overlap")
         return Cm
 
+    def kp_mass(self, E, V, bands, d2h, degen_tol: float = 1e-6):
+        """FP64 k.p sum over states on the device (effm_kp_mass): for each n
+        in `bands`, (1/m*)_ab = sym(d2h_ab) + 2 Re sum_m V_a[n][m] V_b[m][n] /
+        (E_n - E_m) over |E_n - E_m| > degen_tol, m != n.  E: float64 [nb],
+        V: three complex128 [nb, nb] (V_a[m][l] = <m|dH/dk_a|l>), d2h:
+        float64 [len(bands), 3, 3], all on this device; returns a float64
+        [len(bands), 3, 3] device tensor."""
+        import torch
+        dev = torch.device("cuda", self.device)
+        E = E.to(device=dev, dtype=torch.float64).contiguous()
+        nb = E.numel()
+        Vs = [v.to(device=dev, dtype=torch.complex128).contiguous() for v in V]
+        if len(Vs) != 3 or any(tuple(v.shape) != (nb, nb) for v in Vs):
+            raise ValueError(f"V must be three [{nb}, {nb}] matrices")
+        bands = [int(n) for n in bands]
+        d2 = d2h.to(device=dev, dtype=torch.float64).contiguous()
+        if tuple(d2.shape) != (len(bands), 3, 3):
+            raise ValueError("d2h must be [len(bands), 3, 3]")
+        out = torch.empty((len(bands), 3, 3), dtype=torch.float64, device=dev)
+        vp = (C.c_void_p * 3)(*[v.data_ptr() for v in Vs])
+        bs = (C.c_int * max(1, len(bands)))(*bands)
+        self._sync_stream()
+        _lib.check(self._lib.effm_kp_mass(self._ctx, nb, E.data_ptr(), vp, nb, len(bands), bs, d2.data_ptr(),
+                                          float(degen_tol), out.data_ptr(), 0), self._ctx, "effm_kp_mass")
+        return out
+
     def autotune(self, ikpt: int, n_bands: int, prec: str = "fp64"):
         """Time the candidate launch shapes at this k-point and keep the
         fastest for `prec`; returns (bands per pencil workgroup, split-K

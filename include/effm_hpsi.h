@@ -199,6 +199,21 @@ int effm_rayleigh(effm_ctx* ctx, int npw, int nb, const void* psi, int64_t ld,
 int effm_overlap(effm_ctx* ctx, int m, int n, int k, const void* A, int64_t lda,
                  const void* B, int64_t ldb, void* C, int64_t ldc, int prec, int flags);
 
+/* k.p effective-mass sum over states in FP64 on the device, for nsel <= 64
+ * selected bands n = bands[i] (host array) of nb eigenpairs:
+ *   inv_mass[i][a][b] = (d2h[i][a][b] + d2h[i][b][a]) / 2
+ *       + sum_{m != n, |E_n - E_m| > degen_tol}
+ *         Re(V_a[n][m] V_b[m][n] + V_b[n][m] V_a[m][n]) / (E_n - E_m),
+ * i.e. the symmetrised (1/m*)_ab = <n|d2H/dk_a dk_b|n>
+ * + 2 Re sum_m <n|dH_a|m><m|dH_b|n> / (E_n - E_m).  E: double[nb],
+ * V[a]: complex128 [nb][ldv] row-major with V_a[m][l] = <m|dH/dk_a|l>,
+ * d2h / inv_mass: double[nsel][3][3]; all device pointers.  No reference
+ * counterpart: the reference declares the Hessian hooks but leaves them
+ * empty (src/H_psi/hamiltonian_hessian.cpp:48-50; HessHPsi,
+ * src/H_psi/hamiltonian.cppm:84-91). */
+int effm_kp_mass(effm_ctx* ctx, int nb, const double* E, const void* const* V, int64_t ldv, int nsel,
+                 const int* bands, const double* d2h, double degen_tol, double* inv_mass, int flags);
+
 /* Autotuner: times the candidate launch shapes (bands per x-pencil
  * workgroup: 4 / 8 / 16; split-K workgroup target of the projection GEMM:
  * 512 ... 2048) with nb bands at k-point ikpt in precision prec, on
@@ -211,7 +226,7 @@ int effm_autotune(effm_ctx* ctx, int ikpt, int nb, int prec, int* pencil_bands, 
 int effm_set_timing(effm_ctx* ctx, int enable);
 /* Accumulated milliseconds and launch count of a kernel class; names:
  * "plane_g2r", "xpencil_vloc", "plane_r2g", "proj_gemm", "proj_reduce",
- * "proj_apply", "proj_build", "rayleigh". */
+ * "proj_apply", "proj_build", "rayleigh", "overlap", "kp_sum". */
 int effm_get_timing(const effm_ctx* ctx, const char* name, double* ms, int64_t* count);
 int effm_reset_timing(effm_ctx* ctx);
 

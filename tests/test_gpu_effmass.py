@@ -92,3 +92,29 @@ def test_free_electron_mass_on_device():
     m = effmass.effective_mass(H, 0, kp, cell.A, res.X[:8].contiguous(), res.E[:8], [0, 1, 2])
     assert np.abs(m.inv_mass - np.eye(3)[None]).max() < 1e-8
     H.close()
+
+
+def test_device_kp_sum_equals_host_sum():
+    """effm_kp_mass (FP64 sum and denominators on the device) equals the
+    host kp_sum on the same random velocity matrices, eigenvalues with a
+    degenerate pair and d2h, to 1e-13; both orders of the symmetrisation."""
+    rng = np.random.default_rng(3)
+    nb = 77
+    E = np.sort(rng.uniform(-1, 1, nb))
+    E[10] = E[11] = 0.1234                                     # a degenerate pair
+    V = []
+    for _ in range(3):
+        M = rng.standard_normal((nb, nb)) + 1j * rng.standard_normal((nb, nb))
+        V.append(M + M.conj().T)                               # Hermitian velocity matrices
+    bands = [0, 10, 11, 40, 76]
+    d2 = rng.standard_normal((len(bands), 3, 3))
+    wl = synth.custom(*synth.fcc_primitive(10.26), (24, 24, 24), 12.0, [np.zeros(3)], 2, seed=1)
+    H = Hamiltonian(wl.cell)
+    got = H.kp_mass(torch.from_numpy(E).cuda(), [torch.from_numpy(v).cuda() for v in V], bands,
+                    torch.from_numpy(d2).cuda(), 1e-6).cpu().numpy()
+    H.close()
+    for i, n in enumerate(bands):
+        ref, excl = effmass.kp_sum(E, V, d2[i], n, 1e-6)
+        assert np.abs(got[i] - ref).max() <= 1e-13 * np.abs(ref).max(), n
+        if n in (10, 11):
+            assert excl == [10, 11]
