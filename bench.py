@@ -79,6 +79,8 @@ def main():
                     help="time one precision only (profiling runs)")
     ap.add_argument("--multik-steps", type=int, default=3,
                     help="timed steps of the 32-k-point batched H|psi> + dH/dk leg (0 = skip)")
+    ap.add_argument("--facade", type=int, default=1,
+                    help="time the C++ facade's one-band-per-call pattern (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="run a plain device copy of MB megabytes 5 times and exit "
                          "(calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -224,6 +226,9 @@ def main():
                   "mixed_max_rel_err_vs_oracle": float(rel32.max()), "bands_checked": nsb}
 
     H.close()
+    # the C++ facade driven as the reference's callers drive it (one band
+    # per call on host spans) beside its batched and device forms
+    facade = facade_leg([("si2_fcc", synth.build("si2_fcc")), ("si64_cubic", wl)]) if args.facade else None
     # 32 k-points of the 64-atom cell in batched launches, graph replays
     multik = multik_leg(args.multik_steps) if args.multik_steps > 0 else None
 
@@ -274,6 +279,7 @@ def main():
         "kernels": kernels,
         "tuned_launch_shapes": tuned,
         "multi_k": multik,
+        "facade": facade,
         "m_star_latency": _latency_summary(serving_stats),
         "serving": serving_stats,
         "setup_s": setup_s,
@@ -407,6 +413,43 @@ def _cpu_split(oracle, cell, kp, psi, per_band_s):
                                   "pocketfft_local": loc_np * 1e3},
             "value_with_pocketfft_local": 1.0 / (loc_np + nl),
             "split_sample": f"{nb} bands, 1 thread; numpy {np.__version__} pocketfft for the local part"}
+
+
+def write_dump(path, cell, kp, psi):
+    """The binary cell / k-point / band dump effmass_amd/host/facade_bench.cpp reads."""
+    import struct
+    i32 = lambda *v: struct.pack("<%di" % len(v), *v)
+    f64 = lambda a: np.ascontiguousarray(a, dtype="<f8").tobytes()
+    with open(path, "wb") as f:
+        f.write(b"EFFMDUMP" + i32(1) + i32(*cell.n) + f64(np.asarray(cell.A).ravel()) + f64(cell.vloc))
+        f.write(i32(cell.natom) + np.ascontiguousarray(cell.species, "<i4").tobytes() + f64(cell.tau))
+        f.write(i32(len(cell.species_list)))
+        for sp in cell.species_list:
+            f.write(i32(sp.lmax, sp.nbeta) + np.ascontiguousarray(sp.beta_l, "<i4").tobytes() + f64(sp.lam))
+            f.write(i32(sp.nq) + struct.pack("<2d", sp.dq, sp.qmax) + f64(sp.beta_tab))
+            f.write(i32(1) + f64(sp.dbeta_tab) if sp.dbeta_tab is not None else i32(0))
+        f.write(i32(kp.npw) + f64(kp.K) + f64(kp.kin) + np.ascontiguousarray(kp.gidx, "<i4").tobytes()
+                + f64(kp.kfrac))
+        f.write(i32(psi.shape[0]) + np.ascontiguousarray(psi, "<c16").tobytes())
+
+
+def facade_leg(workloads):
+    """ms per band of the C++ facade (effm_hamiltonian.hpp) on host spans,
+    one band per call (the reference's pattern, test_hpsi_eigen.cpp:88) and
+    batched, and on device pointers, for each workload."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "effmass_amd", "host", "_build", "facade_bench")
+    if not os.path.exists(exe):
+        return {"error": "facade_bench not built (__graft_entry__.build())"}
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        for name, wl in workloads:
+            path = os.path.join(d, name + ".dump")
+            write_dump(path, wl.cell, wl.kpts[0], wl.psi[0])
+            r = subprocess.run([exe, path, "3"], capture_output=True, text=True, timeout=600)
+            out[name] = json.loads(r.stdout) if r.returncode == 0 else {"error": r.stderr.strip()[-300:]}
+    return out
 
 
 def _latency_summary(st):

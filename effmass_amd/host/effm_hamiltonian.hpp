@@ -14,10 +14,15 @@
 //   auto g = h.gradient().at_k(ikpt);
 //   g(psi, effm::GradHPsi{dx, dy, dz});   // empty span = skip direction
 //
-// Spans are host memory (complex<double>); the facade stages them through
-// the device.  Status codes are rethrown as the reference's exception
-// classes: std::runtime_error, std::out_of_range, std::domain_error,
-// std::invalid_argument, std::bad_alloc-like std::runtime_error for NOMEM.
+// Spans are host memory, complex<double> (the reference's FP64 path) or
+// complex<float> (the mixed path: FP32 FFT / GEMM, FP64 D contraction); the
+// facade stages them through the device.  Callers whose bands already live
+// in HBM use the device() forms (device pointers, band-major with leading
+// dimensions, either precision, ordered on the stream given to
+// Hamiltonian::set_stream).  Status codes are rethrown as the reference's
+// exception classes: std::runtime_error, std::out_of_range,
+// std::domain_error, std::invalid_argument, std::bad_alloc-like
+// std::runtime_error for NOMEM.
 #pragma once
 
 #include <complex>
@@ -36,6 +41,15 @@ struct GradHPsi {
     std::span<std::complex<double>> y;
     std::span<std::complex<double>> z;
 };
+
+// The same output container for the mixed path (complex64 bands).
+struct GradHPsi32 {
+    std::span<std::complex<float>> x;
+    std::span<std::complex<float>> y;
+    std::span<std::complex<float>> z;
+};
+
+enum class Precision : int { FP64 = EFFM_PREC_FP64, Mixed = EFFM_PREC_MIXED };
 
 struct SpeciesData {
     int lmax = -1;
@@ -114,6 +128,8 @@ public:
     }
 
     effm_ctx* ctx() const { return ctx_; }
+    // order every later device() call on this HIP stream (hipStream_t)
+    void set_stream(void* stream) { throw_status(effm_set_stream(ctx_, stream), ctx_, "set_stream"); }
 
     // -- 1. Callable: H|psi> at fixed k (hamiltonian.cppm:173-205) ---------
     class Callable {
@@ -125,11 +141,22 @@ public:
         }
         void operator()(int n_bands, std::span<const std::complex<double>> psi,
                         std::span<std::complex<double>> hpsi) const {
-            const int ng = dim();
-            if (psi.size() < static_cast<size_t>(n_bands) * ng || hpsi.size() < static_cast<size_t>(n_bands) * ng)
-                throw std::invalid_argument("Callable: spans shorter than n_bands * dim()");
-            throw_status(effm_hpsi(ctx_, ikpt_, n_bands, psi.data(), ng, hpsi.data(), ng, EFFM_PREC_FP64,
-                                   EFFM_F_HOST),
+            host(n_bands, psi.data(), psi.size(), hpsi.data(), hpsi.size(), EFFM_PREC_FP64);
+        }
+        // mixed path on complex64 host spans
+        void operator()(std::span<const std::complex<float>> psi, std::span<std::complex<float>> hpsi) const {
+            (*this)(1, psi, hpsi);
+        }
+        void operator()(int n_bands, std::span<const std::complex<float>> psi,
+                        std::span<std::complex<float>> hpsi) const {
+            host(n_bands, psi.data(), psi.size(), hpsi.data(), hpsi.size(), EFFM_PREC_MIXED);
+        }
+        // device pointers (band b at psi + b * ld complex elements of the
+        // precision's type); hpsi += H psi unless overwrite
+        void device(int n_bands, const void* psi, int64_t ld, void* hpsi, int64_t ld_out, Precision p,
+                    bool overwrite = false) const {
+            throw_status(effm_hpsi(ctx_, ikpt_, n_bands, psi, ld, hpsi, ld_out, static_cast<int>(p),
+                                   overwrite ? EFFM_F_OVERWRITE : 0),
                          ctx_, "H|psi>");
         }
         int dim() const { int n = effm_kpt_dim(ctx_, ikpt_); throw_status(n, ctx_, "Callable::dim"); return n; }
@@ -138,6 +165,12 @@ public:
     private:
         friend class Hamiltonian;
         Callable(effm_ctx* c, int ikpt) : ctx_(c), ikpt_(ikpt) { (void)dim(); }
+        void host(int n_bands, const void* psi, size_t npsi, void* hpsi, size_t nh, int prec) const {
+            const int ng = dim();
+            if (n_bands < 0 || npsi < static_cast<size_t>(n_bands) * ng || nh < static_cast<size_t>(n_bands) * ng)
+                throw std::invalid_argument("Callable: spans shorter than n_bands * dim()");
+            throw_status(effm_hpsi(ctx_, ikpt_, n_bands, psi, ng, hpsi, ng, prec, EFFM_F_HOST), ctx_, "H|psi>");
+        }
         effm_ctx* ctx_ = nullptr;
         int ikpt_ = 0;
     };
@@ -150,17 +183,23 @@ public:
         public:
             Callable() = default;
             void operator()(std::span<const std::complex<double>> psi, GradHPsi out) const {
-                const int ng = dim();
-                if (psi.size() < static_cast<size_t>(ng))
-                    throw std::invalid_argument("Gradient::Callable: psi span shorter than dim()");
-                auto ptr = [&](std::span<std::complex<double>> s) -> void* {
-                    if (s.empty()) return nullptr;
-                    if (s.size() < static_cast<size_t>(ng))
-                        throw std::invalid_argument("GradHPsi span shorter than dim()");
-                    return s.data();
-                };
-                throw_status(effm_dhdk(ctx_, ikpt_, 1, psi.data(), ng, ptr(out.x), ptr(out.y), ptr(out.z), ng,
-                                       EFFM_PREC_FP64, EFFM_F_HOST),
+                host(1, psi.data(), psi.size(), out.x, out.y, out.z, EFFM_PREC_FP64);
+            }
+            // batched bands (band-major, dim() apart) and the mixed path
+            void operator()(int n_bands, std::span<const std::complex<double>> psi, GradHPsi out) const {
+                host(n_bands, psi.data(), psi.size(), out.x, out.y, out.z, EFFM_PREC_FP64);
+            }
+            void operator()(std::span<const std::complex<float>> psi, GradHPsi32 out) const {
+                host(1, psi.data(), psi.size(), out.x, out.y, out.z, EFFM_PREC_MIXED);
+            }
+            void operator()(int n_bands, std::span<const std::complex<float>> psi, GradHPsi32 out) const {
+                host(n_bands, psi.data(), psi.size(), out.x, out.y, out.z, EFFM_PREC_MIXED);
+            }
+            // device pointers; a NULL output skips that direction
+            void device(int n_bands, const void* psi, int64_t ld, void* x, void* y, void* z, int64_t ld_out,
+                        Precision p, bool overwrite = false) const {
+                throw_status(effm_dhdk(ctx_, ikpt_, n_bands, psi, ld, x, y, z, ld_out, static_cast<int>(p),
+                                       overwrite ? EFFM_F_OVERWRITE : 0),
                              ctx_, "dH/dk|psi>");
             }
             int dim() const { int n = effm_kpt_dim(ctx_, ikpt_); throw_status(n, ctx_, "Gradient::dim"); return n; }
@@ -168,6 +207,20 @@ public:
         private:
             friend class Gradient;
             Callable(effm_ctx* c, int ikpt) : ctx_(c), ikpt_(ikpt) { (void)dim(); }
+            template <typename S>
+            void host(int n_bands, const void* psi, size_t npsi, S x, S y, S z, int prec) const {
+                const int ng = dim();
+                const size_t need = static_cast<size_t>(n_bands) * ng;
+                if (n_bands < 0 || npsi < need)
+                    throw std::invalid_argument("Gradient::Callable: psi span shorter than n_bands * dim()");
+                auto ptr = [&](S s) -> void* {
+                    if (s.empty()) return nullptr;
+                    if (s.size() < need) throw std::invalid_argument("GradHPsi span shorter than n_bands * dim()");
+                    return s.data();
+                };
+                throw_status(effm_dhdk(ctx_, ikpt_, n_bands, psi, ng, ptr(x), ptr(y), ptr(z), ng, prec, EFFM_F_HOST),
+                             ctx_, "dH/dk|psi>");
+            }
             effm_ctx* ctx_ = nullptr;
             int ikpt_ = 0;
         };

@@ -1,18 +1,29 @@
 // Exercises the C++ facade (effmass_amd/host/effm_hamiltonian.hpp) the way the
 // reference's tests drive H_psi::Hamiltonian: at_k(ikpt)(psi, hpsi) with
 // accumulate semantics, gradient().at_k(ikpt)(psi, GradHPsi{...}) with a
-// skipped direction, and the exception classes of the error paths.
+// skipped direction, and the exception classes of the error paths; then the
+// mixed-precision host spans and the device-pointer forms.
 // Closed-form case: simple cubic cell, constant V, one s projector
 // beta(q) = exp(-q^2/4) on one atom, so H psi is known analytically.
+// Built twice: with #include of the facade (C++20), and with
+// -DEFFM_USE_MODULE against the C++23 module unit `import H_psi.hip;`.
+#include <hip/hip_runtime_api.h>
+
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <span>
 #include <stdexcept>
 #include <vector>
 
+#ifdef EFFM_USE_MODULE
+import H_psi.hip;
+#else
 #include "effm_hamiltonian.hpp"
+#endif
 
 using cd = std::complex<double>;
+using cf = std::complex<float>;
 
 int main() {
     const double a = 10.0, V0 = 0.1, lam = 0.5, PI = 3.14159265358979323846;
@@ -100,6 +111,64 @@ int main() {
         return 4;
     } catch (const std::domain_error&) {
     }
+    // mixed path on complex64 host spans: closed form within FP32 rounding
+    std::vector<cf> psi32(npw), h32(npw), gx32(npw), gz32(npw);
+    for (int g = 0; g < npw; ++g) psi32[g] = cf((float)psi[g].real(), (float)psi[g].imag());
+    op(std::span<const cf>(psi32), std::span<cf>(h32));
+    h.gradient().at_k(0)(std::span<const cf>(psi32), effm::GradHPsi32{gx32, {}, gz32});
+    double merr = 0, mgerr = 0, gref = 0;
+    for (int g = 0; g < npw; ++g) {
+        cd e = (k.kinetic[g] + V0) * psi[g] + lam * inner * p[g];
+        merr = std::max(merr, std::abs(cd(h32[g]) - e));
+        for (int d : {0, 2}) {
+            cd ge = k.K[3 * g + d] * psi[g] + lam * (dinner[d] * p[g] + inner * dp[d][g]);
+            mgerr = std::max(mgerr, std::abs(cd((d == 0 ? gx32 : gz32)[g]) - ge));
+            gref = std::max(gref, std::abs(ge));
+        }
+    }
+    std::printf("facade mixed H|psi> max err %.3e, dH/dk %.3e\n", merr, mgerr);
+    if (merr > 1e-5 * ref || mgerr > 1e-5 * gref) return 5;
+    // device pointers, FP64, overwrite; two bands (the second is 2 psi)
+    void *dpsi = nullptr, *dh = nullptr, *dgx = nullptr;
+    const size_t bytes = sizeof(cd) * (size_t)npw * 2;
+    if (hipMalloc(&dpsi, bytes) != hipSuccess || hipMalloc(&dh, bytes) != hipSuccess ||
+        hipMalloc(&dgx, bytes) != hipSuccess)
+        return 6;
+    std::vector<cd> two(2 * (size_t)npw), back(2 * (size_t)npw), gback(2 * (size_t)npw);
+    for (int g = 0; g < npw; ++g) two[g] = psi[g], two[npw + g] = 2.0 * psi[g];
+    if (hipMemcpy(dpsi, two.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return 6;
+    op.device(2, dpsi, npw, dh, npw, effm::Precision::FP64, true);
+    h.gradient().at_k(0).device(2, dpsi, npw, dgx, nullptr, nullptr, npw, effm::Precision::FP64, true);
+    if (hipDeviceSynchronize() != hipSuccess) return 6;
+    if (hipMemcpy(back.data(), dh, bytes, hipMemcpyDeviceToHost) != hipSuccess) return 6;
+    if (hipMemcpy(gback.data(), dgx, bytes, hipMemcpyDeviceToHost) != hipSuccess) return 6;
+    double derr = 0, dgerr = 0;
+    for (int bnd = 0; bnd < 2; ++bnd)
+        for (int g = 0; g < npw; ++g) {
+            const double f = bnd + 1.0;
+            cd e = f * ((k.kinetic[g] + V0) * psi[g] + lam * inner * p[g]);
+            cd ge = f * (k.K[3 * g] * psi[g] + lam * (dinner[0] * p[g] + inner * dp[0][g]));
+            derr = std::max(derr, std::abs(back[(size_t)bnd * npw + g] - e));
+            dgerr = std::max(dgerr, std::abs(gback[(size_t)bnd * npw + g] - ge));
+        }
+    (void)hipFree(dpsi);
+    (void)hipFree(dh);
+    (void)hipFree(dgx);
+    std::printf("facade device H|psi> max err %.3e, dH/dk %.3e\n", derr, dgerr);
+    if (derr > 2e-9 * ref || dgerr > 2e-8) return 7;
+#ifdef EFFM_USE_MODULE
+    {
+        // the reference's spelling, exported at global scope by the module
+        Hamiltonian h2(cell);
+        h2.loadKPoint(0, k);
+        std::vector<cd> hp(npw, 0.0);
+        h2.at_k(0)(psi, hp);
+        double e2 = 0;
+        for (int g = 0; g < npw; ++g) e2 = std::max(e2, std::abs(hp[g] - (hpsi[g] - cd(1.0, -1.0))));
+        std::printf("module Hamiltonian vs facade %.3e\n", e2);
+        if (e2 > 1e-12 * ref) return 8;
+    }
+#endif
     std::printf("facade checks passed\n");
     return 0;
 }
