@@ -251,6 +251,18 @@ def main():
                                   progress=_progress(f"serving {key}"))
             st["rate_per_s"] = rate
             serving_stats[key] = st
+        # cost model of the stream: each size class alone (one k-point, 128
+        # bands, 40 Ry), and the rate the device's kernel time allows for the
+        # stream's mix (uniform classes and cutoffs, mean nb 136, mean 2.5
+        # k-points per query), were the kernels of concurrent queries packed
+        # back to back
+        br = serving.class_breakdown(prec="mixed", ecut_ry=40.0, nb=128, reps=2)
+        per_part_ms = float(np.mean([v["kernel_ms"] for v in br.values()])) * (136.0 / 128.0)
+        serving_stats["class_breakdown"] = {"workload": "one query alone per size class: 1 k-point, 128 bands, "
+                                                        "40 Ry, mixed; times in s, kernel time in ms",
+                                            "classes": br,
+                                            "stream_kernel_ms_per_query_estimate": per_part_ms * 2.5,
+                                            "device_bound_queries_per_s_estimate": 1e3 / (per_part_ms * 2.5)}
 
     out = {
         "metric": "Hψ bands/s, 64-atom cell, mixed vs FP64 at 1e-6 Ry; p50/p99 m* query latency",

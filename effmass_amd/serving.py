@@ -156,40 +156,90 @@ def estimate_bytes(q: MassQuery, wl=None, prec="mixed"):
     return int(1.25 * (engine + solver)) + (64 << 20) + (32 << 20)   # + overlap partial sums
 
 
-def run_query(q: MassQuery, prec="mixed", device=0, wl=None, budget=None) -> MassAnswer:
-    """One query, end to end, on the caller's current stream."""
+def run_query(q: MassQuery, prec="mixed", device=0, wl=None, budget=None, stats=None) -> MassAnswer:
+    """One query, end to end, on the caller's current stream.  With a
+    `stats` dict: H apply calls and band applies, eigensolver iterations,
+    setup / solve / mass seconds and (per-kernel HIP-event timing on) the
+    device kernel milliseconds are added to it."""
     t0 = time.perf_counter()
     wl = wl or build_workload(q)
     budget = budget if budget is not None else estimate_bytes(q, wl, prec)
     H = Hamiltonian(wl.cell, device=device, hbm_budget=budget)
+    if stats is not None:
+        H.set_timing(True)
+        for key in ("apply_calls", "band_applies", "iters", "setup_s", "solve_s", "mass_s", "kernel_ms"):
+            stats.setdefault(key, 0)
+        stats["setup_s"] += time.perf_counter() - t0
     band = q.band if q.band >= 0 else q.nb // 2
     inv, en, npws = [], [], []
     try:
         wdt = torch.complex64 if prec == "mixed" else torch.complex128
         tol = 2e-5 if prec == "mixed" else 1e-8
         for ik, kp in enumerate(wl.kpts):
+            t1 = time.perf_counter()
             H.load_kpoint(ik, kp)
             kin = torch.from_numpy(kp.kin).to(f"cuda:{device}")
             nb = min(q.nb, kp.npw)
             X0 = eigensolver.random_start(nb, kp.npw, kin, seed=q.seed + q.k0 + ik, device=f"cuda:{device}")
             op = H.at_k(ik)
+
+            def apply(V, HV):
+                if stats is not None:
+                    stats["apply_calls"] += 1
+                    stats["band_applies"] += V.shape[0]
+                op(V, HV, prec=prec, overwrite=True)
             # Chebyshev-filtered subspace iteration: H applies are cheap next
             # to the dense eigenproblems (measured 1.3-5x faster than LOBPCG
             # on the serving query mix, same eigenvalues to 1e-7 Ha)
-            res = eigensolver.chfsi(lambda V, HV: op(V, HV, prec=prec, overwrite=True), X0, kin,
+            res = eigensolver.chfsi(apply, X0, kin,
                                     nconv=min(nb, band + 2), tol=tol, maxiter=200, work_dtype=wdt,
                                     gram=H.overlap, degree=20, eigh=eigensolver.eigh_host, lincomb=H.lincomb)
+            if stats is not None:
+                torch.cuda.current_stream(device).synchronize()
+                t2 = time.perf_counter()
+                stats["solve_s"] += t2 - t1
+                stats["iters"] += res.iters
             m = effmass.effective_mass(H, ik, kp, wl.cell.A, res.X.contiguous(), res.E, [band], prec=prec)
             inv.append(m.inv_mass[0])
             en.append(float(res.E[band]))
             npws.append(kp.npw)
             H.release_kpoint(ik)
+            if stats is not None:
+                stats["mass_s"] += time.perf_counter() - t2
         torch.cuda.current_stream(device).synchronize()
+        if stats is not None:
+            for name in ("plane_g2r", "xpencil_vloc", "plane_r2g", "proj_gemm", "proj_reduce", "proj_apply",
+                         "proj_build", "overlap", "kp_sum"):
+                stats["kernel_ms"] += H.timing(name)[0]
     finally:
         H.close()
     t1 = time.perf_counter()
     return MassAnswer(qid=q.qid, inv_mass=inv, energies=en, latency=t1 - t0, start=t0, finish=t1, npw=npws,
                       k0=q.k0)
+
+
+def class_breakdown(prec="mixed", device=0, nb=128, ecut_ry=40.0, reps=2, sizes=SIZES):
+    """Per size class (natom), one query of one k-point alone on the device,
+    `reps` times: wall seconds, its solve / mass split, H apply calls, band
+    applies, eigensolver iterations, summed kernel milliseconds (HIP events
+    around each kernel class) and the device idle fraction 1 - kernel / wall
+    of the query.  The cost model of the serving stream."""
+    out = {}
+    rng = np.random.default_rng(7)
+    for natom in sizes:
+        rows = []
+        for r in range(reps):
+            q = MassQuery(qid=r, natom=natom, ecut_ry=ecut_ry, nb=nb, kfracs=[rng.uniform(-0.5, 0.5, 3)],
+                          seed=100 + r)
+            st = {}
+            t0 = time.perf_counter()
+            run_query(q, prec, device, stats=st)
+            st["wall_s"] = time.perf_counter() - t0
+            rows.append(st)
+        avg = {k: float(np.mean([row[k] for row in rows])) for k in rows[0]}
+        avg["device_idle"] = max(0.0, 1.0 - avg["kernel_ms"] * 1e-3 / avg["wall_s"])
+        out[str(natom)] = avg
+    return out
 
 
 def _worker_main(slot, device, prec, tasks, results, host_threads=2):
