@@ -26,15 +26,21 @@ RY_PER_HA = 2.0
 TRAFFIC_PROFILE = "profiles/hpsi_si64_traffic.md"
 
 
-def measured_traffic(kernel, prec):
+def measured_traffic(kernel, prec, pencil_bands=None):
     """HBM bytes per launch of a kernel class at the 64-atom workload, from the
-    rocprofv3 PMC summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
-    plain 1 GiB device copy gives); None if the summary is absent."""
+    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE scaled by what they read
+    for a plain 1 GiB device copy); for the x-pencil kernel the variant with
+    this run's bands per workgroup.  None if the summary is absent."""
     path = os.path.join(ROOT, TRAFFIC_PROFILE)
     try:
         text = open(path).read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
-        return json.loads(block)[kernel][prec]["traffic_bytes"]
+        ent = json.loads(block)[kernel][prec]
+        if pencil_bands is not None:
+            for name, t in ent.get("variants", {}).items():
+                if name.endswith(f", {pencil_bands}>"):
+                    return t
+        return ent["traffic_bytes"]
     except (OSError, IndexError, KeyError, ValueError):
         return None
 
@@ -183,7 +189,8 @@ def main():
               default=None)
     roofline = None                       # needs the per-kernel timing steps (--timing-steps > 0)
     if dom is not None:
-        traffic = measured_traffic(dom["kernel"], "mixed")
+        traffic = measured_traffic(dom["kernel"], "mixed",
+                                   tuned.get("mixed", {}).get("pencil_bands") if dom["kernel"] == "xpencil_vloc" else None)
         roofline = {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
                     "frac": dom["frac"], "traffic": traffic, "kernel": dom["kernel"], "prec": "mixed",
                     "traffic_source": TRAFFIC_PROFILE if traffic is not None else None}
