@@ -246,8 +246,10 @@ def main():
         torch.cuda.empty_cache()
         serving_stats = {"workload": ("Poisson arrivals; cells of 2/8/16/32/64 atoms at 135 bohr^3/atom, "
                                       "ecut 30/40/50 Ry, 16-256 bands, 1-4 k-points; m* of band nb/2 at "
-                                      "every k (Chebyshev-filtered subspace iteration + k.p, mixed "
-                                      "precision); latency = arrival to answer"),
+                                      "every k from the k.p sum over the nb lowest states, all converged "
+                                      "(Chebyshev-filtered subspace iteration from a plane-wave Rayleigh-Ritz "
+                                      "start, max(16, nb/4) buffer rows, mixed precision); latency = "
+                                      "arrival to answer"),
                          "slots": args.serve_slots}
         for key, n, rate, seed in (("rate_50", args.serve_queries, 50.0, 2024),
                                    ("below_capacity", args.serve_light, args.serve_light_rate, 2025)):

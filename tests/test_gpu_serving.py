@@ -57,3 +57,39 @@ def test_footprint_estimate_covers_a_large_cell_with_few_bands():
     q = serving.MassQuery(qid=0, natom=64, ecut_ry=50.0, nb=16, kfracs=[np.array([0.1, -0.2, 0.3])], seed=3)
     a = serving.run_query(q)                  # budget = estimate_bytes(q)
     assert a.error is None and len(a.inv_mass) == 1
+
+
+def test_query_mass_is_converged():
+    """The answer is a property of the query, not of the solver's start: the
+    mixed-precision m* equals the one from a tightly converged FP64 solve
+    (other start vectors, nb + 8 of nb + 16 rows converged to 1e-10) within 1e-4
+    relative, and the band energy within 1e-6 Ry."""
+    import torch
+
+    from effmass_amd import effmass, eigensolver
+    from effmass_amd.hamiltonian import Hamiltonian
+
+    q = serving.MassQuery(qid=0, natom=8, ecut_ry=30.0, nb=24, kfracs=[np.array([0.12, -0.05, 0.2])], seed=11)
+    ans = serving.run_query(q)
+    assert ans.error is None
+    wl = serving.build_workload(q)
+    kp = wl.kpts[0]
+    H = Hamiltonian(wl.cell, device=0)
+    try:
+        H.load_kpoint(0, kp)
+        kin = torch.from_numpy(kp.kin).to("cuda:0")
+        X0 = eigensolver.random_start(q.nb + 16, kp.npw, kin, seed=1234, device="cuda:0")
+        op = H.at_k(0)
+        res = eigensolver.chfsi(lambda V, HV: op(V, HV, prec="fp64", overwrite=True), X0, kin, nconv=q.nb + 8,
+                                tol=1e-10, maxiter=400, gram=H.overlap, degree=20)
+        assert res.converged == q.nb + 8
+        band = q.nb // 2
+        ref = effmass.effective_mass(H, 0, kp, wl.cell.A, res.X[:q.nb].contiguous(), res.E[:q.nb], [band],
+                                     prec="fp64").inv_mass[0]
+    finally:
+        H.close()
+    got = np.asarray(ans.inv_mass[0])
+    rel = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"m* rel diff {rel:.2e}, dE {abs(ans.energies[0] - float(res.E[band])):.2e} Ha", flush=True)
+    assert rel <= 1e-4
+    assert abs(ans.energies[0] - float(res.E[band])) <= 0.5e-6      # Ha (1e-6 Ry)
