@@ -260,14 +260,28 @@ struct ProjBatch {
     ProjSeg<T> s[kMaxSeg];
 };
 
-template <typename T>
+// Workgroup tile 64 x TN: TN = 64 (waves 2 x 2, each 2 x 2 blocks of
+// 16 x 16) or, for N <= 16 (one band per call, small eigensolver blocks),
+// TN = 16 (waves 4 x 1, one block each): a 64-wide tile there spends 64x
+// the MFMA work of the product and made the call MFMA-bound.
+template <int TN> struct GemmWaves {
+    static constexpr int WGC = TN == 64 ? 2 : 1;    // waves along N
+    static constexpr int WGR = 4 / WGC;             // waves along M
+    static constexpr int WR = 64 / (16 * WGR);      // 16-row blocks per wave
+    static constexpr int WC = TN / (16 * WGC);      // 16-column blocks per wave
+    static_assert(TN == 64 || TN == 16, "tile width");
+};
+
+template <typename T, int TN>
 __global__ void __launch_bounds__(256, 2)
 k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0) {
     using C = cx_t<T>;
     using V = typename Acc<T>::type;
+    using W = GemmWaves<TN>;
+    constexpr int WR = W::WR, WC = W::WC;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, kk = lane >> 4;
-    const int nx = (M + 63) / 64, nt = nx * ((N + 63) / 64);
+    const int nx = (M + 63) / 64, nt = nx * ((N + TN - 1) / TN);
     const int L0 = xcd_order(blockIdx.x, gridDim.x);
     // segment-major, then split, then tile: one segment's splits stay together
     const int seg = L0 / (nt * nsplit);
@@ -279,18 +293,18 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
     C* __restrict__ part = sg.part;
     const int64_t lda = sg.lda, ldb = sg.ldb;
     const int K = sg.K, kchunk = sg.kchunk;
-    const int tm0 = (t % nx) * 64, tn0 = (t / nx) * 64;      // workgroup tile
-    const int m0 = tm0 + (wave & 1) * 32;                     // wave tile
-    const int n0 = tn0 + (wave >> 1) * 32;
+    const int tm0 = (t % nx) * 64, tn0 = (t / nx) * TN;      // workgroup tile
+    const int m0 = tm0 + (wave % W::WGR) * (16 * WR);         // wave tile
+    const int n0 = tn0 + (wave / W::WGR) * (16 * WC);
     const int ks = split * kchunk;
     const int ke = min(K, ks + kchunk);
     // Gauss's three-product complex MAC: for conj(P) psi with a = Re P,
     // b = -Im P, c = Re psi, d = Im psi:
     //   k1 += c (a + b),  k2 += a (d - c),  k3 += b (c + d)
     //   Re = k1 - k3,     Im = k1 + k2
-    V k1[2][2], k2[2][2], k3[2][2];
-    for (int r = 0; r < 2; ++r)
-        for (int q = 0; q < 2; ++q) {
+    V k1[WR][WC], k2[WR][WC], k3[WR][WC];
+    for (int r = 0; r < WR; ++r)
+        for (int q = 0; q < WC; ++q) {
             k1[r][q] = V{0, 0, 0, 0};
             k2[r][q] = V{0, 0, 0, 0};
             k3[r][q] = V{0, 0, 0, 0};
@@ -312,9 +326,13 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
     constexpr int LPR = BK / VEC;                     // loading lanes per slab row
     constexpr int RPI = 256 / LPR;                    // rows per load instruction
     constexpr int NLR = 64 / RPI;                     // loads per lane, matrix and slab
+    // B has TN rows: with TN < RPI only the first TN / (RPI / 4) waves load
+    // it (a wave-uniform predicate)
+    constexpr bool BALL = TN >= RPI;
     using LV = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
     __shared__ __align__(16) C sA[2][64 * LDK];
-    __shared__ __align__(16) C sB[2][64 * LDK];
+    __shared__ __align__(16) C sB[2][TN * LDK];
+    const bool bload = BALL || (int)threadIdx.x / LPR < TN;
     // Rows / columns past M / N are clamped onto the last one: they only feed
     // outputs that are never stored, so the loads need no branch.
     const C* ga[NLR];
@@ -326,7 +344,7 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
         const int row = (int)threadIdx.x / LPR + q * RPI;
         const int gr = min(tm0 + row, M - 1);
         ga[q] = (sg.A2 != nullptr && gr >= sg.M1) ? sg.A2 + (int64_t)(gr - sg.M1) * lda : A + (int64_t)gr * lda;
-        gb[q] = B + (int64_t)min(tn0 + row, N - 1) * ldb;
+        gb[q] = B + (int64_t)min(tn0 + (BALL ? row : min(row, TN - 1)), N - 1) * ldb;
         lofs[q] = row * LDK + kofs;
     }
     LV pa[NLR], pb[NLR];
@@ -342,20 +360,26 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
             for (int q = 0; q < NLR; ++q) {
                 const C x = ga[q][kc];
                 pa[q] = kv ? x : C{0, 0};
-                const C y = gb[q][kc];
-                pb[q] = kv ? y : C{0, 0};
+                if (q * RPI < TN && bload) {
+                    const C y = gb[q][kc];
+                    pb[q] = kv ? y : C{0, 0};
+                }
             }
         } else {
 #pragma unroll
             for (int q = 0; q < NLR; ++q) {
                 LV x{}, y{};
+                const bool qb = q * RPI < TN && bload;
                 if (k + VEC <= ke) {
                     x = *reinterpret_cast<const LV*>(ga[q] + k);
-                    y = *reinterpret_cast<const LV*>(gb[q] + k);
+                    if (qb) y = *reinterpret_cast<const LV*>(gb[q] + k);
                 } else if (k < ke) {
-                    const C u = ga[q][k], v = gb[q][k];
+                    const C u = ga[q][k];
                     x.x = u.x; x.y = u.y;
-                    y.x = v.x; y.y = v.y;
+                    if (qb) {
+                        const C v = gb[q][k];
+                        y.x = v.x; y.y = v.y;
+                    }
                 }
                 pa[q] = x;
                 pb[q] = y;
@@ -366,40 +390,43 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
 #pragma unroll
         for (int q = 0; q < NLR; ++q) {
             *reinterpret_cast<LV*>(&sA[buf][lofs[q]]) = pa[q];
-            *reinterpret_cast<LV*>(&sB[buf][lofs[q]]) = pb[q];
+            if (q * RPI < TN && bload) *reinterpret_cast<LV*>(&sB[buf][lofs[q]]) = pb[q];
         }
     };
     // lane (i, kk) takes k = kk*KPL + s of rows / columns i of its blocks
     auto mma = [&](int buf) {
-        C a[2][KPL], b[2][KPL];
+        constexpr int WM = WR > WC ? WR : WC;
+        C a[WR][KPL], b[WC][KPL];
 #pragma unroll
-        for (int r = 0; r < 2; ++r)
+        for (int r = 0; r < WM; ++r)
 #pragma unroll
             for (int s = 0; s < KPL; ++s) {
-                a[r][s] = sA[buf][(m0 - tm0 + r * 16 + i) * LDK + kk * KPL + s];
-                b[r][s] = sB[buf][(n0 - tn0 + r * 16 + i) * LDK + kk * KPL + s];
+                if (r < WR) a[r][s] = sA[buf][(m0 - tm0 + r * 16 + i) * LDK + kk * KPL + s];
+                if (r < WC) b[r][s] = sB[buf][(n0 - tn0 + r * 16 + i) * LDK + kk * KPL + s];
             }
 #pragma unroll
         for (int s = 0; s < KPL; ++s) {
-            T sa[2], sc[2], dc[2];
+            T sa[WR], sc[WC], dc[WC];
 #pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                sa[r] = a[r][s].x - a[r][s].y;             // a + b
-                sc[r] = b[r][s].x + b[r][s].y;             // c + d
-                dc[r] = b[r][s].y - b[r][s].x;             // d - c
+            for (int r = 0; r < WM; ++r) {
+                if (r < WR) sa[r] = a[r][s].x - a[r][s].y;             // a + b
+                if (r < WC) {
+                    sc[r] = b[r][s].x + b[r][s].y;             // c + d
+                    dc[r] = b[r][s].y - b[r][s].x;             // d - c
+                }
             }
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
+            for (int r = 0; r < WR; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) k1[r][q] = mfma16(sa[r], b[q][s].x, k1[r][q]);
+                for (int q = 0; q < WC; ++q) k1[r][q] = mfma16(sa[r], b[q][s].x, k1[r][q]);
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
+            for (int r = 0; r < WR; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) k2[r][q] = mfma16(a[r][s].x, dc[q], k2[r][q]);
+                for (int q = 0; q < WC; ++q) k2[r][q] = mfma16(a[r][s].x, dc[q], k2[r][q]);
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
+            for (int r = 0; r < WR; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) k3[r][q] = mfma16(-a[r][s].y, sc[q], k3[r][q]);
+                for (int q = 0; q < WC; ++q) k3[r][q] = mfma16(-a[r][s].y, sc[q], k3[r][q]);
         }
     };
     if (ks < ke) {
@@ -417,8 +444,8 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
         }
     }
     C* po = part + (int64_t)split * Mtot * N;
-    for (int r = 0; r < 2; ++r)
-        for (int q = 0; q < 2; ++q)
+    for (int r = 0; r < WR; ++r)
+        for (int q = 0; q < WC; ++q)
             for (int v = 0; v < 4; ++v) {
                 int row = m0 + r * 16 + out_row<T>(kk, v);
                 int col = n0 + q * 16 + i;
@@ -457,6 +484,60 @@ __global__ void k_reduce_scale(const ReduceBatch<T> bt, int per_seg, int nsplit,
     out[(int64_t)m * ldo + (idx - (int64_t)m * N)] = cx_t<T>{(T)(re * l), (T)(im * l)};
 }
 
+// The same sum for few outputs and many splits (one band per call: 512
+// outputs over ~150 splits left k_reduce_scale two workgroups of long
+// serial loops): 32 lanes per output, lane l summing splits l, l + 32, ...
+// in FP64, then a fixed-order shuffle tree, so the result is deterministic.
+// grid: nseg x per_seg blocks of 256 (8 outputs each)
+template <typename T>
+__global__ void __launch_bounds__(256) k_reduce_scale_wide(const ReduceBatch<T> bt, int per_seg, int nsplit,
+                                                           int Mtot, int N, const double* __restrict__ lam,
+                                                           int nch, int64_t ldo) {
+    constexpr int LPO = 32;
+    const int seg = blockIdx.x / per_seg;
+    const int l = threadIdx.x % LPO;
+    const int64_t idx = (int64_t)(blockIdx.x - seg * per_seg) * (256 / LPO) + threadIdx.x / LPO;
+    const int64_t tot = (int64_t)Mtot * N;
+    double re = 0.0, im = 0.0;
+    if (idx < tot) {
+        const cx_t<T>* __restrict__ part = bt.part[seg];
+        for (int s = l; s < nsplit; s += LPO) {
+            const cx_t<T> v = part[(int64_t)s * tot + idx];
+            re += (double)v.x;
+            im += (double)v.y;
+        }
+    }
+#pragma unroll
+    for (int off = LPO / 2; off > 0; off >>= 1) {       // every lane takes part: no divergence
+        re += __shfl_down(re, off, LPO);
+        im += __shfl_down(im, off, LPO);
+    }
+    if (idx < tot && l == 0) {
+        const int m = (int)(idx / N);
+        const double lm = lam ? lam[m % nch] : 1.0;
+        bt.out[seg][(int64_t)m * ldo + (idx - (int64_t)m * N)] = cx_t<T>{(T)(re * lm), (T)(im * lm)};
+    }
+}
+
+// out = lambda * sum over the splits, with the kernel that fills the device
+// for the shape (the choice depends on the shape only, so equal shapes give
+// equal bits)
+template <typename T>
+static hipError_t launch_reduce(Ctx& c, const ReduceBatch<T>& bt, int nseg, int nsplit, int Mtot, int N,
+                                const double* lam, int nch, int64_t ldo) {
+    const int64_t tot = (int64_t)Mtot * N;
+    if (tot < 32768 && nsplit >= 16) {
+        const int per = (int)((tot + 7) / 8);
+        hipLaunchKernelGGL((k_reduce_scale_wide<T>), dim3((unsigned)(per * nseg)), dim3(256), 0, c.stream, bt, per,
+                           nsplit, Mtot, N, lam, nch, ldo);
+    } else {
+        const int per = (int)((tot + 255) / 256);
+        hipLaunchKernelGGL((k_reduce_scale<T>), dim3((unsigned)(per * nseg)), dim3(256), 0, c.stream, bt, per,
+                           nsplit, Mtot, N, lam, nch, ldo);
+    }
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 //  GEMM 2: out[n][m] (+)= diag[m] * src[n][m] + sum_k A[k][m] B[k][n]
 //  A: [K][lda] (projectors, G contiguous), B: [K][ldb] (scaled <p|psi>).
@@ -483,14 +564,16 @@ struct ApplyBatch {
     ApplySegK<T> s[kMaxSeg];
 };
 
-template <typename T, bool TWO>
+template <typename T, bool TWO, int TN>
 __global__ void __launch_bounds__(256, 2)
 k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     using C = cx_t<T>;
     using V = typename Acc<T>::type;
+    using W = GemmWaves<TN>;
+    constexpr int WR = W::WR, WC = W::WC;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, kk = lane >> 4;
-    const int ny = (N + 63) / 64;
+    const int ny = (N + TN - 1) / TN;
     const int L0 = xcd_order(blockIdx.x, gridDim.x);
     int seg = 0;
     while (seg + 1 < bt.nseg && L0 >= bt.s[seg + 1].tile0) ++seg;
@@ -505,14 +588,14 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     const C* __restrict__ src = sg.src;
     const int64_t lda = sg.lda, ldb = sg.ldb, ldo = sg.ldo, lds = sg.lds;
     const int M = sg.M;
-    const int tm0 = (L / ny) * 64, tn0 = (L % ny) * 64;      // workgroup tile
-    const int m0 = tm0 + (wave & 1) * 32;                     // wave tile
-    const int n0 = tn0 + (wave >> 1) * 32;
+    const int tm0 = (L / ny) * 64, tn0 = (L % ny) * TN;      // workgroup tile
+    const int m0 = tm0 + (wave % W::WGR) * (16 * WR);         // wave tile
+    const int n0 = tn0 + (wave / W::WGR) * (16 * WC);
     // Gauss's three-product complex MAC with a = Re P, b = Im P, c = Re C,
     // d = Im C: k1 += c (a + b), k2 += a (d - c), k3 += b (c + d).
-    V k1[2][2], k2[2][2], k3[2][2];
-    for (int r = 0; r < 2; ++r)
-        for (int q = 0; q < 2; ++q) {
+    V k1[WR][WC], k2[WR][WC], k3[WR][WC];
+    for (int r = 0; r < WR; ++r)
+        for (int q = 0; q < WC; ++q) {
             k1[r][q] = V{0, 0, 0, 0};
             k2[r][q] = V{0, 0, 0, 0};
             k3[r][q] = V{0, 0, 0, 0};
@@ -538,20 +621,28 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     constexpr int RSTEP = 256 / CPR;                  // slab rows between a lane's loads
     constexpr int NL = BK / RSTEP;                    // loads per lane, matrix and slab
     static_assert(NL * RSTEP == BK, "slab rows must split evenly over the loads");
+    // B slabs are BK x TN: TN = 64 as A; TN = 16 unpadded (a 16-lane read
+    // cycle covers one whole 128- or 256-byte row), loaded by the first
+    // BK * TN / VEC lanes
+    constexpr int LDB = TN == 64 ? LDM : TN;
+    constexpr int CPRB = TN / VEC;
+    constexpr int RSTEPB = 256 / CPRB;
+    constexpr int NLB = (BK + RSTEPB - 1) / RSTEPB;
     using LV = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
     __shared__ __align__(16) C sA[2][BK * LDM];
-    __shared__ __align__(16) C sB[2][BK * LDM];
+    __shared__ __align__(16) C sB[2][BK * LDB];
     const int lrow = threadIdx.x / CPR, lcol = (threadIdx.x % CPR) * VEC;
-    const int ca = tm0 + lcol, cb = tn0 + lcol;       // first column this lane loads
+    const int lrowb = threadIdx.x / CPRB, lcolb = (threadIdx.x % CPRB) * VEC;
+    const int ca = tm0 + lcol, cb = tn0 + lcolb;      // first column this lane loads
     // the k loop runs over nsl slabs of (A, B) and, with TWO, nsl more of
     // (A2, B2); rows past K of a slab load as zero, columns past M / N load
     // as zero (they only feed outputs that are never stored)
     const int nsl = (K + BK - 1) / BK;
     const int ntot = TWO ? 2 * nsl : nsl;
     const C* pa = K > 0 ? A + (int64_t)lrow * lda + ca : nullptr;
-    const C* pb = K > 0 ? B + (int64_t)lrow * ldb + cb : nullptr;
+    const C* pb = K > 0 ? B + (int64_t)lrowb * ldb + cb : nullptr;
     int krow = 0, lt = 0;                             // next slab: its first row, its index
-    LV ra[NL], rb[NL];
+    LV ra[NL], rb[NLB];
     auto load16 = [](const C* p, int col, int ncol) -> LV {
         LV v{};
         if (col + VEC <= ncol) {
@@ -568,14 +659,19 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     auto gload = [&]() {
         if (TWO && lt == nsl) {
             pa = A2 + (int64_t)lrow * lda + ca;
-            pb = B2 + (int64_t)lrow * ldb + cb;
+            pb = B2 + (int64_t)lrowb * ldb + cb;
             krow = 0;
         }
 #pragma unroll
         for (int q = 0; q < NL; ++q) {
             const bool kv = krow + lrow + q * RSTEP < K;
             ra[q] = kv ? load16(pa + (int64_t)(q * RSTEP) * lda, ca, M) : LV{};
-            rb[q] = kv ? load16(pb + (int64_t)(q * RSTEP) * ldb, cb, N) : LV{};
+        }
+#pragma unroll
+        for (int q = 0; q < NLB; ++q) {
+            const int rr = lrowb + q * RSTEPB;
+            const bool kv = (TN == 64 || rr < BK) && krow + rr < K;   // TN = 64: rr < BK always
+            rb[q] = kv ? load16(pb + (int64_t)(q * RSTEPB) * ldb, cb, N) : LV{};
         }
         pa += (int64_t)BK * lda;
         pb += (int64_t)BK * ldb;
@@ -584,9 +680,11 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < NL; ++q) {
-            *reinterpret_cast<LV*>(&sA[buf][(lrow + q * RSTEP) * LDM + lcol]) = ra[q];
-            *reinterpret_cast<LV*>(&sB[buf][(lrow + q * RSTEP) * LDM + lcol]) = rb[q];
+        for (int q = 0; q < NL; ++q) *reinterpret_cast<LV*>(&sA[buf][(lrow + q * RSTEP) * LDM + lcol]) = ra[q];
+#pragma unroll
+        for (int q = 0; q < NLB; ++q) {
+            const int rr = lrowb + q * RSTEPB;
+            if (TN == 64 || rr < BK) *reinterpret_cast<LV*>(&sB[buf][rr * LDB + lcolb]) = rb[q];
         }
     };
     // MFMA k-step s of the slab: lane (i, kk) takes k-row 4 s + kk
@@ -594,31 +692,34 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
 #pragma unroll
         for (int s = 0; s < BK / 4; ++s) {
             const int kr = 4 * s + kk;
-            C a[2], b[2];
+            constexpr int WM = WR > WC ? WR : WC;
+            C a[WR], b[WC];
 #pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                a[r] = sA[buf][kr * LDM + (m0 - tm0) + r * 16 + i];
-                b[r] = sB[buf][kr * LDM + (n0 - tn0) + r * 16 + i];
+            for (int r = 0; r < WM; ++r) {
+                if (r < WR) a[r] = sA[buf][kr * LDM + (m0 - tm0) + r * 16 + i];
+                if (r < WC) b[r] = sB[buf][kr * LDB + (n0 - tn0) + r * 16 + i];
             }
-            T sa[2], sc[2], dc[2];
+            T sa[WR], sc[WC], dc[WC];
 #pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                sa[r] = a[r].x + a[r].y;
-                sc[r] = b[r].x + b[r].y;
-                dc[r] = b[r].y - b[r].x;
+            for (int r = 0; r < WM; ++r) {
+                if (r < WR) sa[r] = a[r].x + a[r].y;
+                if (r < WC) {
+                    sc[r] = b[r].x + b[r].y;
+                    dc[r] = b[r].y - b[r].x;
+                }
             }
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
+            for (int r = 0; r < WR; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) k1[r][q] = mfma16(sa[r], b[q].x, k1[r][q]);
+                for (int q = 0; q < WC; ++q) k1[r][q] = mfma16(sa[r], b[q].x, k1[r][q]);
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
+            for (int r = 0; r < WR; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) k2[r][q] = mfma16(a[r].x, dc[q], k2[r][q]);
+                for (int q = 0; q < WC; ++q) k2[r][q] = mfma16(a[r].x, dc[q], k2[r][q]);
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
+            for (int r = 0; r < WR; ++r)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) k3[r][q] = mfma16(a[r].y, sc[q], k3[r][q]);
+                for (int q = 0; q < WC; ++q) k3[r][q] = mfma16(a[r].y, sc[q], k3[r][q]);
         }
     };
     if (K > 0) {
@@ -635,8 +736,8 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
             buf ^= 1;
         }
     }
-    for (int r = 0; r < 2; ++r)
-        for (int q = 0; q < 2; ++q) {
+    for (int r = 0; r < WR; ++r)
+        for (int q = 0; q < WC; ++q) {
             const int n = n0 + q * 16 + i;
             if (n >= N) continue;
             if constexpr (sizeof(T) == 4) {
@@ -701,7 +802,7 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
 // Resident workgroups of the projection GEMM on the whole device.
 template <typename T>
 static int gemm_proj_slots(Ctx& c) {
-    const void* fn = reinterpret_cast<const void*>(&k_gemm_proj<T>);
+    const void* fn = reinterpret_cast<const void*>(&k_gemm_proj<T, 64>);
     auto it = c.occupancy.find(fn);
     if (it == c.occupancy.end()) {
         int per_cu = 0;
@@ -768,8 +869,12 @@ static hipError_t gemm_proj(Ctx& c, const ApplySeg* segs, int nseg, const cx_t<T
                              partial + (size_t)i * nsplit * Mtot * nb, (int64_t)npw, segs[i].ld, npw, kchunk,
                              A2 ? A2[i] : nullptr, A2 ? M1 : M};
     }
+    // nb <= 16: one 64 x 16 tile column (same grid as one 64 x 64 one)
     dim3 grid(((M + 63) / 64) * ((nb + 63) / 64) * nsplit * nseg);
-    hipLaunchKernelGGL((k_gemm_proj<T>), grid, dim3(256), 0, c.stream, bt, M, nb, nsplit, Mtot, row0);
+    if (nb <= 16)
+        hipLaunchKernelGGL((k_gemm_proj<T, 16>), grid, dim3(256), 0, c.stream, bt, M, nb, nsplit, Mtot, row0);
+    else
+        hipLaunchKernelGGL((k_gemm_proj<T, 64>), grid, dim3(256), 0, c.stream, bt, M, nb, nsplit, Mtot, row0);
     return hipGetLastError();
 }
 
@@ -782,11 +887,8 @@ static hipError_t reduce_scale(Ctx& c, int nseg, cx_t<T>* partial, int nsplit, i
         bt.part[i] = partial + (size_t)i * nsplit * Mtot * nb;
         bt.out[i] = cbuf + (size_t)i * Mtot * nb;
     }
-    const int64_t tot = (int64_t)Mtot * nb;
-    const int per = (int)((tot + 255) / 256);
-    hipLaunchKernelGGL((k_reduce_scale<T>), dim3((unsigned)(per * nseg)), dim3(256), 0, c.stream, bt, per,
-                       nsplit, Mtot, nb, static_cast<const double*>(c.lam64.ptr), c.nch(), (int64_t)nb);
-    return hipGetLastError();
+    return launch_reduce<T>(c, bt, nseg, nsplit, Mtot, nb, static_cast<const double*>(c.lam64.ptr), c.nch(),
+                            (int64_t)nb);
 }
 
 // out_s (+)= diag_s * src_s + A_s B_s for the segments with out_s != NULL
@@ -803,11 +905,20 @@ static hipError_t gemm_apply(Ctx& c, int nseg, const ApplySegK<T>* in, int nb, i
         tiles += ((s.M + 63) / 64) * ny;
     }
     if (bt.nseg == 0 || tiles == 0) return hipSuccess;
-    // all segments of a call either have the second product or none
-    if (bt.s[0].A2)
-        hipLaunchKernelGGL((k_gemm_apply<T, true>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, acc ? 1 : 0);
-    else
-        hipLaunchKernelGGL((k_gemm_apply<T, false>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, acc ? 1 : 0);
+    // all segments of a call either have the second product or none; nb <=
+    // 16 takes the 64 x 16 tile (the same tile count as 64 x 64 there)
+    const int a = acc ? 1 : 0;
+    if (bt.s[0].A2) {
+        if (nb <= 16)
+            hipLaunchKernelGGL((k_gemm_apply<T, true, 16>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
+        else
+            hipLaunchKernelGGL((k_gemm_apply<T, true, 64>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
+    } else {
+        if (nb <= 16)
+            hipLaunchKernelGGL((k_gemm_apply<T, false, 16>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
+        else
+            hipLaunchKernelGGL((k_gemm_apply<T, false, 64>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
+    }
     return hipGetLastError();
 }
 
@@ -946,18 +1057,17 @@ hipError_t launch_overlap(Ctx& c, int m, int n, int k, const void* A, int64_t ld
                          k, kchunk, nullptr, m};
     if (timed) timer_begin(c);
     dim3 grid(((m + 63) / 64) * ((n + 63) / 64) * nsplit);
-    hipLaunchKernelGGL((k_gemm_proj<T>), grid, dim3(256), 0, c.stream, bt, m, n, nsplit, m, 0);
+    if (n <= 16)
+        hipLaunchKernelGGL((k_gemm_proj<T, 16>), grid, dim3(256), 0, c.stream, bt, m, n, nsplit, m, 0);
+    else
+        hipLaunchKernelGGL((k_gemm_proj<T, 64>), grid, dim3(256), 0, c.stream, bt, m, n, nsplit, m, 0);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
         ReduceBatch<T> rb{};
         rb.nseg = 1;
         rb.part[0] = static_cast<const Cx*>(partial);
         rb.out[0] = static_cast<Cx*>(C);
-        const int64_t tot = (int64_t)m * n;
-        const int per = (int)((tot + 255) / 256);
-        hipLaunchKernelGGL((k_reduce_scale<T>), dim3((unsigned)per), dim3(256), 0, c.stream, rb, per, nsplit, m, n,
-                           static_cast<const double*>(nullptr), 1, ldc);
-        e = hipGetLastError();
+        e = launch_reduce<T>(c, rb, 1, nsplit, m, n, nullptr, 1, ldc);
     }
     if (timed) timer_end(c, "overlap");
     return e;

@@ -19,7 +19,8 @@ def _torch():
     return torch
 
 
-@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (37, 101, 777), (256, 242, 24001), (64, 64, 0), (130, 3, 5000)])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (37, 101, 777), (256, 242, 24001), (64, 64, 0), (130, 3, 5000),
+                                   (512, 16, 30001), (512, 17, 30001), (200, 1, 40000)])
 def test_overlap_matches_matmul(m, n, k):
     torch = _torch()
     wl = workload("si2_fcc")
