@@ -331,7 +331,7 @@ def chfsi(apply, X0: torch.Tensor, kin: torch.Tensor, nconv: int | None = None, 
 
 def plane_wave_start(apply, kin: torch.Tensor, rows: int, m: int | None = None, work_dtype=torch.complex64,
                      eigh=None) -> torch.Tensor:
-    """Start vectors from a Rayleigh-Ritz step in the m (default 2 rows)
+    """Start vectors from a Rayleigh-Ritz step in the m (default 1.25 rows)
     plane waves of lowest kinetic energy: H is applied to those unit
     vectors (one block of m band applies in the work precision), the m x m
     matrix <G|H|G'> is diagonalised in FP64 and its `rows` lowest
@@ -339,9 +339,11 @@ def plane_wave_start(apply, kin: torch.Tensor, rows: int, m: int | None = None, 
     (complex128, rows x npw).  The low-lying states of a smooth potential
     live mostly on the low-|k+G| plane waves, so the filtered iteration
     starts near the wanted subspace: on the 8-64 atom serving cells it
-    needs about half the band applies of a random start."""
+    needs 25-45% fewer band applies than a random start.  (m = rows took
+    7 instead of 4 filter steps on the 64-atom cell; 1.25 rows needs as few
+    band applies as 2 rows, at a host eigensolve half as long.)"""
     npw = int(kin.numel())
-    m = min(npw, max(rows, m if m is not None else 2 * rows))
+    m = min(npw, max(rows, m if m is not None else rows + rows // 4))
     idx = torch.argsort(kin.to(torch.float64), stable=True)[:m]
     ar = torch.arange(m, device=kin.device)
     E = torch.zeros(m, npw, dtype=work_dtype, device=kin.device)

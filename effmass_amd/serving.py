@@ -206,7 +206,9 @@ def run_query(q: MassQuery, prec="mixed", device=0, wl=None, budget=None, stats=
             # agrees with an FP64 solve to 1e-6-1e-4); buffer rows beyond nb
             # are filtered with the largest degree a wanted row still needs.
             # Measured on 8-64 atom cells (128 bands, two k-points each): the
-            # plane-wave start halves the band applies of a random start, a
+            # plane-wave start (a Rayleigh-Ritz step in the 1.25 rows
+            # lowest-kinetic plane waves) needs 25-45% fewer band applies
+            # than a random start, with 3-4 instead of 5-7 filter steps; a
             # buffer of nb/4 rows saves 7-20% of the kernel time against nb/8.
             res = eigensolver.chfsi(apply, X0, kin,
                                     nconv=nb, tol=tol, maxiter=200, work_dtype=wdt,
