@@ -187,10 +187,13 @@ def diamond_cubic(a: float, rep: int = 1):
     return A, np.array(pos)
 
 
-def g_sphere(A: np.ndarray, n, kfrac, ecut_ry: float):
+def g_sphere(A: np.ndarray, n, kfrac, ecut_ry: float, order: str = "kinetic"):
     """All centred integer G in the FFT box with |k+G|^2 <= ecut (Ry units:
-    |K|^2 in bohr^-2 equals the kinetic energy in Ry).  Sorted by |K|^2 then
-    by index (stable), the usual PWmat/QE sphere order."""
+    |K|^2 in bohr^-2 equals the kinetic energy in Ry).  order "kinetic":
+    sorted by |K|^2 then by index (stable), the usual PWmat/QE sphere order;
+    "plane": by FFT-grid plane (g1 mod n1), row (g2 mod n2) and column
+    (g3 mod n3), so each plane's and row's plane waves are consecutive (the
+    device scatter / gather then reads and writes contiguous runs)."""
     B = 2.0 * np.pi * np.linalg.inv(A).T
     axes = [np.arange(-(m // 2), m - m // 2) for m in n]
     g1, g2, g3 = np.meshgrid(*axes, indexing="ij")
@@ -198,12 +201,18 @@ def g_sphere(A: np.ndarray, n, kfrac, ecut_ry: float):
     K = (g + np.asarray(kfrac)[None, :]) @ B
     k2 = np.einsum("ij,ij->i", K, K)
     keep = np.nonzero(k2 <= ecut_ry * (1.0 + 1e-14))[0]
-    order = keep[np.argsort(k2[keep], kind="stable")]
-    K = K[order]
+    if order == "kinetic":
+        sel = keep[np.argsort(k2[keep], kind="stable")]
+    elif order == "plane":
+        gm = np.mod(g[keep], np.asarray(n)[None, :])
+        sel = keep[np.argsort((gm[:, 0] * n[1] + gm[:, 1]) * n[2] + gm[:, 2], kind="stable")]
+    else:
+        raise ValueError(f"unknown sphere order {order!r}")
+    K = K[sel]
     return KPoint(kfrac=np.asarray(kfrac, dtype=np.float64),
                   K=np.ascontiguousarray(K),
                   kin=0.5 * np.einsum("ij,ij->i", K, K),
-                  gidx=np.ascontiguousarray(g[order].astype(np.int32)))
+                  gidx=np.ascontiguousarray(g[sel].astype(np.int32)))
 
 
 def make_vloc(A, n, tau, rng: SplitMix64, amp_ry=-2.0, sigma=1.0,
@@ -314,10 +323,10 @@ def mp_grid(m1, m2, m3):
 
 
 def build(name: str, nb: int | None = None, nk: int | None = None, seed: int | None = None,
-          with_psi: bool = True, two_species: bool = False) -> Workload:
+          with_psi: bool = True, two_species: bool = False, order: str = "kinetic") -> Workload:
     """Build the named workload (WORKLOAD_SEEDS).  nb / nk shrink the band
     or k-point count for quick parity cases; the cell, grid and cutoff never
-    change."""
+    change; `order` is the sphere order of g_sphere."""
     if name not in WORKLOAD_SEEDS:
         raise ValueError(f"unknown workload {name!r}")
     seed = WORKLOAD_SEEDS[name] if seed is None else seed
@@ -346,15 +355,17 @@ def build(name: str, nb: int | None = None, nk: int | None = None, seed: int | N
     vloc = make_vloc(A, n, tau, rng)
     cell = Cell(A=A, n=n, vloc=vloc, tau=np.ascontiguousarray(tau, dtype=np.float64),
                 species=species, species_list=species_list)
-    kpts = [g_sphere(A, n, k, ecut) for k in ks]
+    kpts = [g_sphere(A, n, k, ecut, order) for k in ks]
     wl = Workload(name=name, cell=cell, kpts=kpts, nb=nbands, ecut_ry=ecut, seed=seed)
     if with_psi:
         wl.psi = [make_psi(kp, nbands, rng) for kp in kpts]
     return wl
 
 
-def custom(A, tau, n, ecut_ry, kfracs, nb, seed, species_idx=None, species_list=None, with_psi=True):
-    """A workload for an arbitrary cell (used by the serving loop)."""
+def custom(A, tau, n, ecut_ry, kfracs, nb, seed, species_idx=None, species_list=None, with_psi=True,
+           order="kinetic"):
+    """A workload for an arbitrary cell (used by the serving loop); `order`
+    is the sphere order of g_sphere."""
     rng = SplitMix64(seed)
     if species_list is None:
         species_list = [make_species(rng)]
@@ -363,7 +374,7 @@ def custom(A, tau, n, ecut_ry, kfracs, nb, seed, species_idx=None, species_list=
     cell = Cell(A=np.asarray(A, float), n=tuple(n), vloc=vloc,
                 tau=np.ascontiguousarray(tau, dtype=np.float64), species=species,
                 species_list=species_list)
-    kpts = [g_sphere(cell.A, n, k, ecut_ry) for k in kfracs]
+    kpts = [g_sphere(cell.A, n, k, ecut_ry, order) for k in kfracs]
     wl = Workload(name="custom", cell=cell, kpts=kpts, nb=nb, ecut_ry=ecut_ry, seed=seed)
     if with_psi:
         wl.psi = [make_psi(kp, nb, rng) for kp in kpts]
