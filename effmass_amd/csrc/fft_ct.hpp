@@ -116,7 +116,11 @@ __device__ __forceinline__ void ct_stage_rows(cx_t<T>* buf, const cx_t<T>* tw, c
             const int r = t - li * NBF;
             const int b = r / L;
             const int k1 = r - b * L;
-            butterfly<P, S, DIT, T>(buf, rows[li] * N + b * LT + k1, L, Pfa<N>::on ? 0 : k1, TSTEP, tw);
+            // a 24-bit multiply: as a 32-bit one it became a 64-bit mad whose
+            // unused high addend half was allocated to the destination of an
+            // in-flight load, and the wait for that load stalled the row pass
+            butterfly<P, S, DIT, T>(buf, (int)__umul24((unsigned)rows[li], (unsigned)N) + b * LT + k1, L,
+                                    Pfa<N>::on ? 0 : k1, TSTEP, tw);
         }
     }
 }

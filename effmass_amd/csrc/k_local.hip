@@ -343,23 +343,27 @@ static SplitShape split_shape(int n2, int n3) {
 //        thread (indices during the row FFT, psi values during the columns);
 //   GPF  R2G: this item's first GPF gather entries per thread (indices
 //        during the first column stage, psi / kin / hpsi after it);
-//   YPF  R2G: column-stage inputs u < YPF of the next plane, during the
-//        gather (the rest at the top of the item);
+//   YPF  R2G: column-stage inputs u < YPF of the next plane, issued before
+//        this plane's row pass (in flight through the row pass and the
+//        gather; the rest at the top of the item); the next row list is
+//        fetched after the gather;
 //   G2RW / R2GW  waves per SIMD the register allocation must allow;
 //   BLOCK_CAP  largest workgroup: FP32 planes (65 KB of LDS) run two
 //        workgroups of 8 waves per CU, each thread taking up to two
 //        butterflies per stage, where 15-wave workgroups would need <= 64
 //        VGPRs for a second one.
 // Entries past SPF / GPF per thread (the densest planes) load directly.
-// Measured on the 90^3 grid (256 bands): FP64 R2G 420 -> 388 us with GPF 1;
-// FP32 G2R 242 -> 220 us with 512-thread workgroups, FP32 R2G 236 -> 228 us
-// at two workgroups per CU (either shape); YPF > 0 spills in FP64 and did
-// not pay in FP32; SPF changes G2R by 1-3%.
+// Measured on the 90^3 grid (256 bands): FP64 R2G 420 -> 388 us with GPF 1,
+// 402 -> 385 us with YPF 4 (and 605 -> 534 us on a 48^3 grid the sphere
+// fills; YPF 6-8 fit only without GPF and were slower); FP32 G2R 242 ->
+// 220 us with 512-thread workgroups, FP32 R2G 236 -> 228 us at two
+// workgroups per CU (either shape); YPF > 0 did not pay in FP32; SPF
+// changes G2R by 1-3%.
 template <typename T>
 struct PlaneTune;
 template <>
 struct PlaneTune<double> {
-    static constexpr int G2RW = 1, R2GW = 1, SPF = 1, GPF = 1, YPF = 0, BLOCK_CAP = 1024;
+    static constexpr int G2RW = 1, R2GW = 1, SPF = 1, GPF = 1, YPF = 4, BLOCK_CAP = 1024;
 };
 template <>
 struct PlaneTune<float> {
@@ -496,14 +500,19 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
     auto plane_of = [&](const PlaneItem<T>& q) {
         return static_cast<const C*>(q.sg->work) + (int64_t)q.b * q.sg->nplanes * PS + q.p * q.sg->wsp;
     };
-    auto fetch = [&](const PlaneItem<T>& q) {
-        ct_cols_first_load<N, WP, BLOCK, T, 0, YP>(plane_of(q), y, q.sg->wsr);
+    auto fetch_rows = [&](const PlaneItem<T>& q) {
         const int r0 = q.sg->row_ptr[q.p];
         nrows = q.sg->row_ptr[q.p + 1] - r0;
         if (threadIdx.x < nrows) srow = q.sg->rows[r0 + threadIdx.x];
     };
+    auto fetch_y = [&](const PlaneItem<T>& q) {
+        ct_cols_first_load<N, WP, BLOCK, T, 0, YP>(plane_of(q), y, q.sg->wsr);
+    };
     int item = blockIdx.x;
-    if (item < nitems) fetch(plane_item(bt, item));
+    if (item < nitems) {
+        fetch_rows(plane_item(bt, item));
+        fetch_y(plane_item(bt, item));
+    }
     for (; item < nitems; item += gridDim.x) {
         const PlaneItem<T> cur = plane_item(bt, item);
         const LocalSeg<T>& sg = *cur.sg;
@@ -542,9 +551,9 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
                 gh[f] = accumulate ? hs[gi[f]] : C{0, 0};
             }
         }
-        ct_fft_rows<N, -1, false, BLOCK, T>(plane, tw, srows, nr);        // along k, gathered rows
         const int nxt = item + gridDim.x;
-        if (nxt < nitems) fetch(plane_item(bt, nxt));
+        if (nxt < nitems) fetch_y(plane_item(bt, nxt));
+        ct_fft_rows<N, -1, false, BLOCK, T>(plane, tw, srows, nr);        // along k, gathered rows
 #pragma unroll
         for (int f = 0; f < GPF; ++f) {
             if (gi[f] < 0) continue;
@@ -567,6 +576,7 @@ k_plane_r2g_ct(const LocalBatch<T> bt, int nitems, T scale, int accumulate,
             }
             hs[ig] = C{(T)re, (T)im};
         }
+        if (nxt < nitems) fetch_rows(plane_item(bt, nxt));
         __syncthreads();
     }
 }
