@@ -561,24 +561,38 @@ struct ApplySegK {
 template <typename T>
 struct ApplyBatch {
     int nseg;
+    // tiles [0, full) are whole 64 x TN tiles, one workgroup each; with TN =
+    // 64 the tiles [full, total) run as four 64 x 16 tiles each, after the
+    // whole ones (the per-CU remainder of the tile count, see gemm_apply)
+    int full;
     ApplySegK<T> s[kMaxSeg];
 };
 
+// Slab depth and LDS row pitch of the apply GEMM.  FP64: 12 k-rows per slab
+// and unpadded rows, 48 KB, so three workgroups share a CU (measured on the
+// 64-atom apply: 8 rows 680 us, 16 rows 640 us at two workgroups per CU, 12
+// rows 590 us); FP32: 16 rows (wider slabs halve the workgroups per CU,
+// narrower add barriers).  Row pitch: a 16-byte read cycle (8 lanes) covers
+// 128 contiguous bytes of one k-row, so FP64 needs no padding; FP32 rows are
+// 64 elements + 128 bytes, consecutive k-rows start 32 banks apart and the
+// two 16-lane halves of an 8-byte read cycle do not collide.
+template <typename T> struct ApplyShape {
+    static constexpr int BK = sizeof(T) == 8 ? 12 : 16;
+    static constexpr int LDM = sizeof(T) == 8 ? 64 : 64 + 128 / (int)sizeof(cx_t<T>);
+};
+
+// One 64 x TN tile of hpsi (+)= A B (+ A2 B2): rows tm0.., columns tn0.. of
+// the segment's output; sA / sB are the workgroup's double-buffered LDS
+// slabs (room for TN = 64).
 template <typename T, bool TWO, int TN>
-__global__ void __launch_bounds__(256, 2)
-k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
+__device__ __forceinline__ void apply_tile(const ApplySegK<T>& sg, int tm0, int tn0, int N, int K, int accumulate,
+                                           cx_t<T>* sA, cx_t<T>* sB) {
     using C = cx_t<T>;
     using V = typename Acc<T>::type;
     using W = GemmWaves<TN>;
     constexpr int WR = W::WR, WC = W::WC;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, kk = lane >> 4;
-    const int ny = (N + TN - 1) / TN;
-    const int L0 = xcd_order(blockIdx.x, gridDim.x);
-    int seg = 0;
-    while (seg + 1 < bt.nseg && L0 >= bt.s[seg + 1].tile0) ++seg;
-    const ApplySegK<T>& sg = bt.s[seg];
-    const int L = L0 - sg.tile0;
     const C* __restrict__ A = sg.A;
     const C* __restrict__ B = sg.B;
     const C* __restrict__ A2 = sg.A2;
@@ -588,7 +602,6 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     const C* __restrict__ src = sg.src;
     const int64_t lda = sg.lda, ldb = sg.ldb, ldo = sg.ldo, lds = sg.lds;
     const int M = sg.M;
-    const int tm0 = (L / ny) * 64, tn0 = (L % ny) * TN;      // workgroup tile
     const int m0 = tm0 + (wave % W::WGR) * (16 * WR);         // wave tile
     const int n0 = tn0 + (wave / W::WGR) * (16 * WC);
     // Gauss's three-product complex MAC with a = Re P, b = Im P, c = Re C,
@@ -600,22 +613,13 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
             k2[r][q] = V{0, 0, 0, 0};
             k3[r][q] = V{0, 0, 0, 0};
         }
-    // Slabs of BK k-rows of A (64 consecutive m) and of B (64 consecutive n)
+    // Slabs of BK k-rows of A (64 consecutive m) and of B (TN consecutive n)
     // are staged through LDS, double-buffered.  Every global load is 16
     // bytes (one complex128, or two consecutive complex64), consecutive
     // lanes on consecutive addresses, so one instruction covers whole cache
     // lines; the row pointers advance by BK rows per slab (no per-element
-    // 64-bit index arithmetic).  FP64: 12 k-rows per slab and unpadded rows,
-    // 48 KB, so three workgroups share a CU (measured on the 64-atom apply:
-    // 8 rows 680 us, 16 rows 640 us at two workgroups per CU, 12 rows
-    // 590 us); FP32: 16 rows (wider slabs halve the workgroups per CU,
-    // narrower add barriers)
-    constexpr int BK = sizeof(T) == 8 ? 12 : 16;
-    // row stride: a 16-byte read cycle (8 lanes) covers 128 contiguous bytes
-    // of one k-row, so FP64 needs no padding; FP32 rows are 64 elements +
-    // 128 bytes, consecutive k-rows start 32 banks apart and the two 16-lane
-    // halves of an 8-byte read cycle do not collide
-    constexpr int LDM = sizeof(T) == 8 ? 64 : 64 + 128 / (int)sizeof(C);
+    // 64-bit index arithmetic).
+    constexpr int BK = ApplyShape<T>::BK, LDM = ApplyShape<T>::LDM;
     constexpr int VEC = 16 / (int)sizeof(C);          // complex per 16-byte load
     constexpr int CPR = 64 / VEC;                     // loading lanes per slab row
     constexpr int RSTEP = 256 / CPR;                  // slab rows between a lane's loads
@@ -628,9 +632,11 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     constexpr int CPRB = TN / VEC;
     constexpr int RSTEPB = 256 / CPRB;
     constexpr int NLB = (BK + RSTEPB - 1) / RSTEPB;
+    // FP32 loads each slab two slabs ahead (a second register set, 16 more
+    // VGPRs; measured on the 64-atom apply: 2-3% faster); FP64 one ahead (a
+    // second set would cost a workgroup per CU)
+    constexpr int DEPTH = sizeof(T) == 4 ? 2 : 1;
     using LV = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
-    __shared__ __align__(16) C sA[2][BK * LDM];
-    __shared__ __align__(16) C sB[2][BK * LDB];
     const int lrow = threadIdx.x / CPR, lcol = (threadIdx.x % CPR) * VEC;
     const int lrowb = threadIdx.x / CPRB, lcolb = (threadIdx.x % CPRB) * VEC;
     const int ca = tm0 + lcol, cb = tn0 + lcolb;      // first column this lane loads
@@ -642,7 +648,7 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     const C* pa = K > 0 ? A + (int64_t)lrow * lda + ca : nullptr;
     const C* pb = K > 0 ? B + (int64_t)lrowb * ldb + cb : nullptr;
     int krow = 0, lt = 0;                             // next slab: its first row, its index
-    LV ra[NL], rb[NLB];
+    LV ra[DEPTH][NL], rb[DEPTH][NLB];
     auto load16 = [](const C* p, int col, int ncol) -> LV {
         LV v{};
         if (col + VEC <= ncol) {
@@ -656,7 +662,8 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
         }
         return v;
     };
-    auto gload = [&]() {
+    // the next slab into register set d
+    auto gload = [&](int d) {
         if (TWO && lt == nsl) {
             pa = A2 + (int64_t)lrow * lda + ca;
             pb = B2 + (int64_t)lrowb * ldb + cb;
@@ -665,30 +672,34 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
 #pragma unroll
         for (int q = 0; q < NL; ++q) {
             const bool kv = krow + lrow + q * RSTEP < K;
-            ra[q] = kv ? load16(pa + (int64_t)(q * RSTEP) * lda, ca, M) : LV{};
+            ra[d][q] = kv ? load16(pa + (int64_t)(q * RSTEP) * lda, ca, M) : LV{};
         }
 #pragma unroll
         for (int q = 0; q < NLB; ++q) {
             const int rr = lrowb + q * RSTEPB;
             const bool kv = (TN == 64 || rr < BK) && krow + rr < K;   // TN = 64: rr < BK always
-            rb[q] = kv ? load16(pb + (int64_t)(q * RSTEPB) * ldb, cb, N) : LV{};
+            rb[d][q] = kv ? load16(pb + (int64_t)(q * RSTEPB) * ldb, cb, N) : LV{};
         }
         pa += (int64_t)BK * lda;
         pb += (int64_t)BK * ldb;
         krow += BK;
         ++lt;
     };
-    auto lstore = [&](int buf) {
+    auto lstore = [&](int buf, int d) {
+        C* a = sA + buf * (BK * LDM);
+        C* b = sB + buf * (BK * LDB);
 #pragma unroll
-        for (int q = 0; q < NL; ++q) *reinterpret_cast<LV*>(&sA[buf][(lrow + q * RSTEP) * LDM + lcol]) = ra[q];
+        for (int q = 0; q < NL; ++q) *reinterpret_cast<LV*>(&a[(lrow + q * RSTEP) * LDM + lcol]) = ra[d][q];
 #pragma unroll
         for (int q = 0; q < NLB; ++q) {
             const int rr = lrowb + q * RSTEPB;
-            if (TN == 64 || rr < BK) *reinterpret_cast<LV*>(&sB[buf][rr * LDB + lcolb]) = rb[q];
+            if (TN == 64 || rr < BK) *reinterpret_cast<LV*>(&b[rr * LDB + lcolb]) = rb[d][q];
         }
     };
     // MFMA k-step s of the slab: lane (i, kk) takes k-row 4 s + kk
     auto mma = [&](int buf) {
+        const C* a_s = sA + buf * (BK * LDM);
+        const C* b_s = sB + buf * (BK * LDB);
 #pragma unroll
         for (int s = 0; s < BK / 4; ++s) {
             const int kr = 4 * s + kk;
@@ -696,8 +707,8 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
             C a[WR], b[WC];
 #pragma unroll
             for (int r = 0; r < WM; ++r) {
-                if (r < WR) a[r] = sA[buf][kr * LDM + (m0 - tm0) + r * 16 + i];
-                if (r < WC) b[r] = sB[buf][kr * LDB + (n0 - tn0) + r * 16 + i];
+                if (r < WR) a[r] = a_s[kr * LDM + (m0 - tm0) + r * 16 + i];
+                if (r < WC) b[r] = b_s[kr * LDB + (n0 - tn0) + r * 16 + i];
             }
             T sa[WR], sc[WC], dc[WC];
 #pragma unroll
@@ -722,18 +733,40 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
                 for (int q = 0; q < WC; ++q) k3[r][q] = mfma16(a[r].y, sc[q], k3[r][q]);
         }
     };
-    if (K > 0) {
-        gload();
-        lstore(0);
-        __syncthreads();
-        int buf = 0;
-        for (int t = 0; t < ntot; ++t) {
-            const bool more = t + 1 < ntot;
-            if (more) gload();                 // next slab in flight during the MFMAs
-            mma(buf);
-            if (more) lstore(buf ^ 1);
+    if (ntot > 0) {
+        if constexpr (DEPTH == 1) {
+            gload(0);
+            lstore(0, 0);
             __syncthreads();
-            buf ^= 1;
+            int buf = 0;
+            for (int t = 0; t < ntot; ++t) {
+                const bool more = t + 1 < ntot;
+                if (more) gload(0);            // next slab in flight during the MFMAs
+                mma(buf);
+                if (more) lstore(buf ^ 1, 0);
+                __syncthreads();
+                buf ^= 1;
+            }
+        } else {
+            // slab t sits in LDS buffer t % 2 and, before that, in register
+            // set t % 2; two slabs are in flight during the MFMAs (the loop
+            // is unrolled by two so that the register sets are fixed)
+            gload(0);
+            if (ntot > 1) gload(1);
+            lstore(0, 0);
+            __syncthreads();
+            for (int t = 0; t < ntot; t += 2) {
+                if (t + 2 < ntot) gload(0);
+                mma(0);
+                if (t + 1 < ntot) lstore(1, 1);
+                __syncthreads();
+                if (t + 1 < ntot) {
+                    if (t + 3 < ntot) gload(1);
+                    mma(1);
+                    if (t + 2 < ntot) lstore(0, 0);
+                    __syncthreads();
+                }
+            }
         }
     }
     for (int r = 0; r < WR; ++r)
@@ -797,6 +830,42 @@ k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
                 *o = C{(T)re, (T)im};
             }
         }
+}
+
+// GEMM 2: out[n][m] (+)= sum_k A[k][m] B[k][n] (+ A2 B2), per segment, in
+// 64 x TN tiles.  Workgroups [0, full) take whole tiles (1D grid renumbered
+// so that the column tiles of one row tile run on one XCD); with TN = 64 the
+// workgroups past `full` each take a quarter (64 x 16) of one of the
+// remaining tiles, so those tiles spread over four times as many CUs.
+template <typename T, bool TWO, int TN>
+__global__ void __launch_bounds__(256, 2)
+k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
+    using C = cx_t<T>;
+    constexpr int BK = ApplyShape<T>::BK, LDM = ApplyShape<T>::LDM;
+    constexpr int LDB = TN == 64 ? LDM : TN;
+    __shared__ __align__(16) C sA[2 * BK * LDM];
+    __shared__ __align__(16) C sB[2 * BK * LDB];
+    const int ny = (N + TN - 1) / TN;
+    int L0, sub = -1;
+    if (TN == 64 && (int)blockIdx.x >= bt.full) {
+        const int b = (int)blockIdx.x - bt.full;
+        L0 = bt.full + b / 4;
+        sub = b % 4;
+    } else {
+        L0 = xcd_order(blockIdx.x, bt.full);
+    }
+    int seg = 0;
+    while (seg + 1 < bt.nseg && L0 >= bt.s[seg + 1].tile0) ++seg;
+    const ApplySegK<T>& sg = bt.s[seg];
+    const int L = L0 - sg.tile0;
+    const int tm0 = (L / ny) * 64, tn0 = (L % ny) * TN;
+    if constexpr (TN == 64) {
+        if (sub >= 0) {
+            if (tn0 + 16 * sub < N) apply_tile<T, TWO, 16>(sg, tm0, tn0 + 16 * sub, N, K, accumulate, sA, sB);
+            return;
+        }
+    }
+    apply_tile<T, TWO, TN>(sg, tm0, tn0, N, K, accumulate, sA, sB);
 }
 
 // Resident workgroups of the projection GEMM on the whole device.
@@ -908,17 +977,26 @@ static hipError_t gemm_apply(Ctx& c, int nseg, const ApplySegK<T>* in, int nb, i
     // all segments of a call either have the second product or none; nb <=
     // 16 takes the 64 x 16 tile (the same tile count as 64 x 64 there)
     const int a = acc ? 1 : 0;
-    if (bt.s[0].A2) {
-        if (nb <= 16)
+    bt.full = tiles;
+    if (nb <= 16) {
+        if (bt.s[0].A2)
             hipLaunchKernelGGL((k_gemm_apply<T, true, 16>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
         else
-            hipLaunchKernelGGL((k_gemm_apply<T, true, 64>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
-    } else {
-        if (nb <= 16)
             hipLaunchKernelGGL((k_gemm_apply<T, false, 16>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
-        else
-            hipLaunchKernelGGL((k_gemm_apply<T, false, 64>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
+        return hipGetLastError();
     }
+    // Workgroups on one CU share its MFMA pipes, so the kernel lasts as long
+    // as the CU with the most tiles: with tiles = q * CUs + r, r small, r CUs
+    // take a whole tile more than the rest.  Those r tiles run as 4 r
+    // quarter tiles after the others instead (measured on the 64-atom apply,
+    // 2312 tiles = 9 * 256 + 8: the 8 extra tiles cost 6.6% of the kernel).
+    const int r = tiles % c.ncu;
+    if (r > 0 && 4 * r <= c.ncu) bt.full = tiles - r;
+    const int grid = bt.full + 4 * (tiles - bt.full);
+    if (bt.s[0].A2)
+        hipLaunchKernelGGL((k_gemm_apply<T, true, 64>), dim3(grid), dim3(256), 0, c.stream, bt, nb, K, a);
+    else
+        hipLaunchKernelGGL((k_gemm_apply<T, false, 64>), dim3(grid), dim3(256), 0, c.stream, bt, nb, K, a);
     return hipGetLastError();
 }
 

@@ -271,11 +271,33 @@ def class_breakdown(prec="mixed", device=0, nb=128, ecut_ry=40.0, reps=2, sizes=
     return out
 
 
-def _worker_main(slot, device, prec, tasks, results, host_threads=2):
+# hipDeviceSchedule* flags of hipSetDeviceFlags: how a host thread waits in a
+# HIP synchronisation
+_SYNC_FLAGS = {"spin": 1, "yield": 2, "blocking": 4}
+
+
+def _set_sync_mode(device, sync):
+    """Sets how this process's host threads wait for the device: "spin"
+    (busy-wait), "yield" or "blocking" (sleep until the device signals).
+    Must run before anything in the process touches the device."""
+    if sync == "auto":
+        return
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    rc = hip.hipSetDevice(ctypes.c_int(device))
+    if rc == 0:
+        rc = hip.hipSetDeviceFlags(ctypes.c_uint(_SYNC_FLAGS[sync]))
+    if rc != 0:
+        raise RuntimeError(f"hipSetDeviceFlags({sync}) failed with HIP error {rc}")
+
+
+def _worker_main(slot, device, prec, tasks, results, host_threads=2, sync="auto"):
     """A serving slot: its own process, HIP context and stream; its host
     LAPACK work (the small Rayleigh-Ritz eigenproblems) uses host_threads
-    threads so that the slots share the host cores."""
+    threads so that the slots share the host cores; `sync` sets how its
+    threads wait for the device (_set_sync_mode)."""
     try:
+        _set_sync_mode(device, sync)
         import threadpoolctl
         threadpoolctl.threadpool_limits(host_threads)
         torch.set_num_threads(host_threads)
@@ -317,9 +339,11 @@ class MassServer:
     MAX_SLOTS = 15
 
     def __init__(self, device=0, hbm_budget=None, slots=4, prec="mixed", start_timeout=600.0,
-                 split_kpoints=True, host_threads=2):
+                 split_kpoints=True, host_threads=2, sync="auto"):
         if not 1 <= slots <= self.MAX_SLOTS:
             raise ValueError(f"slots must be 1..{self.MAX_SLOTS}")
+        if sync not in ("auto",) + tuple(_SYNC_FLAGS):
+            raise ValueError(f"sync must be one of auto, {', '.join(_SYNC_FLAGS)}")
         self.device = device
         self.prec = prec
         self.slots = slots
@@ -329,7 +353,7 @@ class MassServer:
         self._tasks = [ctx.Queue() for _ in range(slots)]
         self._results = ctx.Queue()
         self._procs = [ctx.Process(target=_worker_main,
-                                   args=(i, device, prec, self._tasks[i], self._results, host_threads),
+                                   args=(i, device, prec, self._tasks[i], self._results, host_threads, sync),
                                    daemon=True, name=f"mass-slot-{i}") for i in range(slots)]
         for p in self._procs:
             p.start()
@@ -496,12 +520,12 @@ class MassServer:
 
 
 def serve(queries, device=0, slots=4, prec="mixed", hbm_budget=None, realtime=True, timeout=None, progress=None,
-          split_kpoints=True, host_threads=2):
+          split_kpoints=True, host_threads=2, sync="auto"):
     """Replay `queries` at their arrival times (realtime) or all at once;
     return (answers by qid, stats).  progress(done, total) is called about
     once a second while waiting."""
     srv = MassServer(device=device, hbm_budget=hbm_budget, slots=slots, prec=prec, split_kpoints=split_kpoints,
-                     host_threads=host_threads)
+                     host_threads=host_threads, sync=sync)
     try:
         t0 = time.perf_counter()
         for q in sorted(queries, key=lambda x: x.arrival):
