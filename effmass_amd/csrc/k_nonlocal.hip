@@ -272,8 +272,13 @@ template <int TN> struct GemmWaves {
     static_assert(TN == 64 || TN == 16, "tile width");
 };
 
+// Workgroups per CU the projection's register allocation must allow: FP32
+// keeps four (its LDS allows four) with the second register set of slabs in
+// flight; FP64 is held to three by its accumulators.
+template <typename T> struct ProjTune { static constexpr int MINB = sizeof(T) == 4 ? 4 : 2; };
+
 template <typename T, int TN>
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(256, ProjTune<T>::MINB)
 k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0) {
     using C = cx_t<T>;
     using V = typename Acc<T>::type;
@@ -347,9 +352,13 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
         gb[q] = B + (int64_t)min(tn0 + (BALL ? row : min(row, TN - 1)), N - 1) * ldb;
         lofs[q] = row * LDK + kofs;
     }
-    LV pa[NLR], pb[NLR];
+    // FP32 loads each slab two slabs ahead (a second register set at the
+    // same four workgroups per CU; measured on the 64-atom projection: 271
+    // -> 259 us), FP64 one ahead
+    constexpr int DEPTH = sizeof(T) == 4 ? 2 : 1;
+    LV pa[DEPTH][NLR], pb[DEPTH][NLR];
     // k past the split end loads as zero
-    auto gload = [&](int kb) {
+    auto gload = [&](int kb, int d) {
         const int k = kb + kofs;
         if constexpr (VEC == 1) {
             // clamped address, zeroed operands: no branch (and no scratch
@@ -359,10 +368,10 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
 #pragma unroll
             for (int q = 0; q < NLR; ++q) {
                 const C x = ga[q][kc];
-                pa[q] = kv ? x : C{0, 0};
+                pa[d][q] = kv ? x : C{0, 0};
                 if (q * RPI < TN && bload) {
                     const C y = gb[q][kc];
-                    pb[q] = kv ? y : C{0, 0};
+                    pb[d][q] = kv ? y : C{0, 0};
                 }
             }
         } else {
@@ -381,16 +390,16 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
                         y.x = v.x; y.y = v.y;
                     }
                 }
-                pa[q] = x;
-                pb[q] = y;
+                pa[d][q] = x;
+                pb[d][q] = y;
             }
         }
     };
-    auto lstore = [&](int buf) {
+    auto lstore = [&](int buf, int d) {
 #pragma unroll
         for (int q = 0; q < NLR; ++q) {
-            *reinterpret_cast<LV*>(&sA[buf][lofs[q]]) = pa[q];
-            if (q * RPI < TN && bload) *reinterpret_cast<LV*>(&sB[buf][lofs[q]]) = pb[q];
+            *reinterpret_cast<LV*>(&sA[buf][lofs[q]]) = pa[d][q];
+            if (q * RPI < TN && bload) *reinterpret_cast<LV*>(&sB[buf][lofs[q]]) = pb[d][q];
         }
     };
     // lane (i, kk) takes k = kk*KPL + s of rows / columns i of its blocks
@@ -430,17 +439,38 @@ k_gemm_proj(const ProjBatch<T> bt, int M, int N, int nsplit, int Mtot, int row0)
         }
     };
     if (ks < ke) {
-        gload(ks);
-        lstore(0);
-        __syncthreads();
-        int buf = 0;
-        for (int kb = ks; kb < ke; kb += BK) {
-            const bool more = kb + BK < ke;
-            if (more) gload(kb + BK);          // next slab in flight during the MFMAs
-            mma(buf);
-            if (more) lstore(buf ^ 1);
+        if constexpr (DEPTH == 1) {
+            gload(ks, 0);
+            lstore(0, 0);
             __syncthreads();
-            buf ^= 1;
+            int buf = 0;
+            for (int kb = ks; kb < ke; kb += BK) {
+                const bool more = kb + BK < ke;
+                if (more) gload(kb + BK, 0);   // next slab in flight during the MFMAs
+                mma(buf);
+                if (more) lstore(buf ^ 1, 0);
+                __syncthreads();
+                buf ^= 1;
+            }
+        } else {
+            // the slab at kb sits in LDS buffer and register set
+            // ((kb - ks) / BK) % 2; two slabs in flight during the MFMAs
+            gload(ks, 0);
+            if (ks + BK < ke) gload(ks + BK, 1);
+            lstore(0, 0);
+            __syncthreads();
+            for (int kb = ks; kb < ke; kb += 2 * BK) {
+                if (kb + 2 * BK < ke) gload(kb + 2 * BK, 0);
+                mma(0);
+                if (kb + BK < ke) lstore(1, 1);
+                __syncthreads();
+                if (kb + BK < ke) {
+                    if (kb + 3 * BK < ke) gload(kb + 3 * BK, 1);
+                    mma(1);
+                    if (kb + 2 * BK < ke) lstore(0, 0);
+                    __syncthreads();
+                }
+            }
         }
     }
     C* po = part + (int64_t)split * Mtot * N;
