@@ -238,6 +238,8 @@ def main():
     facade = facade_leg([("si2_fcc", synth.build("si2_fcc")), ("si64_cubic", wl)]) if args.facade else None
     # 32 k-points of the 64-atom cell in batched launches, graph replays
     multik = multik_leg(args.multik_steps) if args.multik_steps > 0 else None
+    # the vendor library on GEMMs of the same shapes, for reference
+    libgemm = library_gemm_leg(npw, nb, cell.nchannels(), {k["kernel"] + "/" + k["prec"]: k["avg_us"] for k in kernels})
 
     # serving: Poisson(50/s) effective-mass queries over 2-64 atom cells
     serving_stats = None
@@ -300,12 +302,56 @@ def main():
         "kernels": kernels,
         "tuned_launch_shapes": tuned,
         "multi_k": multik,
+        "library_gemm": libgemm,
         "facade": facade,
         "m_star_latency": _latency_summary(serving_stats),
         "serving": serving_stats,
         "setup_s": setup_s,
     }
     print(json.dumps(out), flush=True)
+
+
+def library_gemm_leg(npw, nb, nch, ours, reps=5):
+    """torch.matmul (rocBLAS cgemm / zgemm) on random operands of the shapes
+    of the two projector GEMMs of the step, beside this library's kernels:
+    the apply out[b][g] += sum_c C[c][b] P[c][g] (M = npw, N = nb, K = nch)
+    and the projection <p|psi> = conj(P) psi^T (M = nch, N = nb, K = npw).
+    Algorithmic flops 8 M N K as for the library's kernels."""
+    import torch
+    dev = "cuda:0"
+    out = {"workload": f"random operands, apply {npw} x {nb} x {nch}, projection {nch} x {nb} x {npw}",
+           "library": "torch.matmul / addmm (rocBLAS)"}
+    flops = 8.0 * npw * nb * nch
+    g = torch.Generator(device=dev).manual_seed(5)
+    for prec, dt, peak in (("fp64", torch.complex128, 78.6), ("mixed", torch.complex64, 157.3)):
+        P = torch.randn(nch, npw, dtype=dt, device=dev, generator=g)
+        Ct = torch.randn(nb, nch, dtype=dt, device=dev, generator=g)          # C^T, contiguous
+        psiT = torch.randn(npw, nb, dtype=dt, device=dev, generator=g)        # psi^T, contiguous
+        h = torch.zeros(nb, npw, dtype=dt, device=dev)
+        Pc = P.conj().resolve_conj()
+
+        def timed(fn):
+            for _ in range(2):
+                fn()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) * 1e3 / reps
+        t_apply = timed(lambda: h.addmm_(Ct, P))
+        t_proj = timed(lambda: torch.matmul(Pc, psiT))
+        ent = {}
+        for name, t in (("apply", t_apply), ("projection", t_proj)):
+            key = "proj_apply" if name == "apply" else "proj_gemm"
+            mine = ours.get(f"{key}/{prec}")
+            ent[name] = {"library_us": t, "library_frac": flops / t / 1e6 / peak, "ours_us": mine,
+                         "speedup": (t / mine) if mine else None}
+        out[prec] = ent
+        del P, Ct, psiT, h, Pc
+    torch.cuda.empty_cache()
+    return out
 
 
 def multik_leg(steps, warmup=1):
