@@ -340,7 +340,7 @@ static SplitShape split_shape(int n2, int n3) {
 // psi).  The kernels load inputs ahead, a level at a time, behind the FFT
 // stages (per precision, PlaneTune):
 //   SPF  G2R: the next item's row list and its first SPF scatter entries per
-//        thread (indices during the row FFT, psi values during the columns);
+//        thread (G2R_EARLY below: when they load);
 //   GPF  R2G: this item's first GPF gather entries per thread (indices
 //        during the first column stage, psi / kin / hpsi after it);
 //   YPF  R2G: column-stage inputs u < YPF of the next plane, issued before
@@ -357,18 +357,27 @@ static SplitShape split_shape(int n2, int n3) {
 // 402 -> 385 us with YPF 4 (and 605 -> 534 us on a 48^3 grid the sphere
 // fills; YPF 6-8 fit only without GPF and were slower); FP32 G2R 242 ->
 // 220 us with 512-thread workgroups, FP32 R2G 236 -> 228 us at two
-// workgroups per CU (either shape); YPF > 0 did not pay in FP32; SPF
-// changes G2R by 1-3%.
+// workgroups per CU (either shape); YPF > 0 did not pay in FP32 (YPF 4
+// with the early inputs waited for before the gather: 217-218 vs 219-222
+// us; YPF 9 leaves one workgroup per CU: 261 us); SPF changes G2R by 1-3%
+// with the FP64 order (the FP32 one: below).
 template <typename T>
 struct PlaneTune;
 template <>
 struct PlaneTune<double> {
     static constexpr int G2RW = 1, R2GW = 1, SPF = 1, GPF = 1, YPF = 4, BLOCK_CAP = 1024;
+    static constexpr bool G2R_EARLY = false;
 };
 template <>
 struct PlaneTune<float> {
     static constexpr int G2RW = 1, R2GW = 1, SPF = 2, GPF = 2, YPF = 0, BLOCK_CAP = 512;
+    static constexpr bool G2R_EARLY = true;
 };
+
+// Waits until every vector memory access this wave issued has completed
+// (s_waitcnt vmcnt(0), other counters unconstrained).  A real waitcnt
+// instruction: the compiler's own wait insertion counts it.
+__device__ __forceinline__ void vmem_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // Flat item -> (segment, plane, band).
 template <typename T>
@@ -426,19 +435,39 @@ k_plane_g2r_ct(const LocalBatch<T> bt, int nitems, const cx_t<T>* __restrict__ t
         for (int f = 0; f < SPF; ++f)
             if (spos[f] >= 0) sval[f] = ps[sig[f]];
     };
+    // EARLY (FP32): the next item's row list, scatter indices and psi values
+    // load during this item's row passes and are waited for before its
+    // workspace stores, and the row list goes to LDS then.  Vector loads
+    // and stores share one counter, so a wait for a load issued after the
+    // stores (the row-list and scatter uses at the top of the next item
+    // compiled to one) waits for every store of the wave (the scatter
+    // entries past SPF per thread still load after the stores: SPF = 3,
+    // which covers every plane of the 90^3 grid, was slower, 195-198 vs
+    // 184-186 us).  FP64 keeps the later loads: its 130 KB of stores
+    // per item take longer to drain than an item's FFT stages, so any
+    // wait inside the next item stalls on them, and the stalls come out
+    // fewest with the loads at the end of the item.  Measured on the 90^3
+    // grid (256 bands): FP32 G2R 213 -> 185 us; the same order in FP64
+    // 328 -> 418 us.
+    constexpr bool EARLY = PlaneTune<T>::G2R_EARLY;
     int item = blockIdx.x;
     if (item < nitems) {
         const PlaneItem<T> q = plane_item(bt, item);
         fetch_rows(q);
         fetch_index(q);
         fetch_values(q);
+        if constexpr (EARLY) {
+            vmem_wait_all();
+            if (threadIdx.x < nrows) srows[threadIdx.x] = srow;
+        }
     }
     for (; item < nitems; item += gridDim.x) {
         const PlaneItem<T> cur = plane_item(bt, item);
         const int nxt = item + gridDim.x;
         const PlaneItem<T> q = plane_item(bt, nxt < nitems ? nxt : item);
         const int nr = nrows;
-        if (threadIdx.x < nr) srows[threadIdx.x] = srow;
+        if constexpr (!EARLY)
+            if (threadIdx.x < nr) srows[threadIdx.x] = srow;
 #pragma unroll
         for (int it = 0; it < NZ; ++it) {
             const int i = threadIdx.x + it * BLOCK;
@@ -456,18 +485,33 @@ k_plane_g2r_ct(const LocalBatch<T> bt, int nitems, const cx_t<T>* __restrict__ t
                 plane[sg.ent_pos[e]] = ps[sg.ent_ig[e]];
         }
         __syncthreads();
-        if (nxt < nitems) fetch_rows(q);
+        if (nxt < nitems) {
+            fetch_rows(q);
+            if constexpr (EARLY) fetch_index(q);
+        }
         // along k, rows with data
         ct_stage_rows<N, Plan2<N>::P1, 1, +1, true, BLOCK, T>(plane, tw, srows, nr);
         __syncthreads();
-        if (nxt < nitems) fetch_index(q);
+        if (nxt < nitems) {
+            if constexpr (EARLY)
+                fetch_values(q);
+            else
+                fetch_index(q);
+        }
         ct_stage_rows<N, Plan2<N>::P2, Plan2<N>::P1, +1, true, BLOCK, T>(plane, tw, srows, nr);
         __syncthreads();
         // along j (columns): first stage in LDS, the last one stores the rows
         // (pad columns of FP32 rows of 128-byte pitch carry junk)
         ct_stage<N, Plan2<N>::P1, 1, +1, true, N, 1, N, BLOCK, T>(plane, tw);
         __syncthreads();
-        if (nxt < nitems) fetch_values(q);
+        if (nxt < nitems) {
+            if constexpr (EARLY) {
+                vmem_wait_all();
+                if (threadIdx.x < nrows) srows[threadIdx.x] = srow;   // this item's rows are no longer read
+            } else {
+                fetch_values(q);
+            }
+        }
         ct_cols_last_to_global<N, WP, +1, BLOCK, T>(
             plane, tw, cur.sg->work + (int64_t)cur.b * cur.sg->nplanes * PS + cur.p * cur.sg->wsp, cur.sg->wsr);
         __syncthreads();
