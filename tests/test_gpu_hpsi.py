@@ -496,3 +496,41 @@ def test_narrow_band_counts(nb):
     torch.cuda.synchronize()
     assert band_rel(h1.cpu().numpy(), hf[:nb].cpu().numpy()) <= 1e-13
     H.close()
+
+
+def test_apply_gemm_quarter_tiles():
+    """The apply GEMM's per-CU remainder tiles as 64x16 quarter tiles: 150
+    bands on the 8-atom cell give 101 x 3 = 303 tiles of 64 x 64, i.e. 256
+    whole ones and 47 run as quarter tiles, the last column tile partial
+    (bands 128-149).  H|psi> and dH/dk from one 150-band call against one
+    call per band (the separate 64 x 16 tile kernel) at 1e-13 (FP64) / 1e-6
+    (mixed), and the partial column tile's bands against the oracle
+    (FP64 1e-12, mixed 2e-5)."""
+    torch = _torch()
+    nb = 150
+    wl = workload("si8_cubic", nb=nb)
+    kp = wl.kpts[0]
+    H = Hamiltonian(wl.cell)
+    H.load_kpoint(0, kp)
+    tail = slice(128, nb)
+    ref = oracle.hpsi(wl.cell, kp, wl.psi[0][tail], nthreads=8)
+    gref = oracle.dhdk(wl.cell, kp, wl.psi[0][tail])
+    for prec, dt, tol_call, tol_ref in (("fp64", torch.complex128, 1e-13, 1e-12),
+                                        ("mixed", torch.complex64, 1e-6, 2e-5)):
+        p = torch.from_numpy(wl.psi[0]).to(dt).cuda()
+        h = torch.zeros_like(p)
+        H.at_k(0)(p, h, prec=prec)
+        g = [torch.zeros_like(p) for _ in range(3)]
+        H.gradient().at_k(0)(p, GradHPsi(*g), prec=prec)
+        h1 = torch.zeros_like(p)
+        g1 = [torch.zeros_like(p) for _ in range(3)]
+        for b in range(nb):
+            H.at_k(0)(p[b:b + 1], h1[b:b + 1], prec=prec)
+            H.gradient().at_k(0)(p[b:b + 1], GradHPsi(*(x[b:b + 1] for x in g1)), prec=prec)
+        torch.cuda.synchronize()
+        hb, h1n = h.cpu().numpy(), h1.cpu().numpy()
+        e_call = max(band_rel(hb, h1n), max(band_rel(g[a].cpu().numpy(), g1[a].cpu().numpy()) for a in range(3)))
+        e_ref = max(band_rel(hb[tail], ref), max(band_rel(g[a].cpu().numpy()[tail], gref[a]) for a in range(3)))
+        log(f"quarter tiles {prec}: batch vs per-band {e_call:.2e}, tail vs oracle {e_ref:.2e}")
+        assert e_call <= tol_call and e_ref <= tol_ref
+    H.close()
