@@ -170,11 +170,13 @@ def estimate_bytes(q: MassQuery, wl=None, prec="mixed"):
     return int(1.25 * (engine + solver)) + (64 << 20) + (32 << 20)   # + overlap partial sums
 
 
-def run_query(q: MassQuery, prec="mixed", device=0, wl=None, budget=None, stats=None) -> MassAnswer:
+def run_query(q: MassQuery, prec="mixed", device=0, wl=None, budget=None, stats=None,
+              rr_host: bool = False) -> MassAnswer:
     """One query, end to end, on the caller's current stream.  With a
     `stats` dict: H apply calls and band applies, eigensolver iterations,
     setup / solve / mass seconds and (per-kernel HIP-event timing on) the
-    device kernel milliseconds are added to it."""
+    device kernel milliseconds are added to it.  rr_host: the Rayleigh-Ritz
+    eigenproblems on the host (LAPACK) instead of the device."""
     t0 = time.perf_counter()
     wl = wl or build_workload(q)
     budget = budget if budget is not None else estimate_bytes(q, wl, prec)
@@ -185,6 +187,11 @@ def run_query(q: MassQuery, prec="mixed", device=0, wl=None, budget=None, stats=
             stats.setdefault(key, 0)
         stats["setup_s"] += time.perf_counter() - t0
     band = q.band if q.band >= 0 else q.nb // 2
+    # the small Rayleigh-Ritz eigenproblems in FP64 on the device (rocSOLVER
+    # through torch): on a 150-query burst with 8 slots 9.8 queries/s against
+    # 9.2 with host LAPACK (2 threads per slot), eigenvalues equal to 4e-14;
+    # rr_host=True keeps them on the host
+    rr_eigh = eigensolver.eigh_host if rr_host else None
     inv, en, npws = [], [], []
     try:
         wdt = torch.complex64 if prec == "mixed" else torch.complex128
@@ -202,7 +209,7 @@ def run_query(q: MassQuery, prec="mixed", device=0, wl=None, budget=None, stats=
                     stats["apply_calls"] += 1
                     stats["band_applies"] += V.shape[0]
                 op(V, HV, prec=prec, overwrite=True)
-            X0 = eigensolver.plane_wave_start(apply, kin, rows, work_dtype=wdt, eigh=eigensolver.eigh_host)
+            X0 = eigensolver.plane_wave_start(apply, kin, rows, work_dtype=wdt, eigh=rr_eigh)
             # Chebyshev-filtered subspace iteration: H applies are cheap next
             # to the dense eigenproblems (measured 1.3-5x faster than LOBPCG
             # on the serving query mix, same eigenvalues to 1e-7 Ha).  Every
@@ -218,7 +225,7 @@ def run_query(q: MassQuery, prec="mixed", device=0, wl=None, budget=None, stats=
             # buffer of nb/4 rows saves 7-20% of the kernel time against nb/8.
             res = eigensolver.chfsi(apply, X0, kin,
                                     nconv=nb, tol=tol, maxiter=200, work_dtype=wdt,
-                                    gram=H.overlap, degree=30, eigh=eigensolver.eigh_host, lincomb=H.lincomb,
+                                    gram=H.overlap, degree=30, eigh=rr_eigh, lincomb=H.lincomb,
                                     buffer_degree="wanted")
             if stats is not None:
                 torch.cuda.current_stream(device).synchronize()
