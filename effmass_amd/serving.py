@@ -171,12 +171,12 @@ def estimate_bytes(q: MassQuery, wl=None, prec="mixed"):
 
 
 def run_query(q: MassQuery, prec="mixed", device=0, wl=None, budget=None, stats=None,
-              rr_host: bool = False) -> MassAnswer:
+              rr_host: bool = True) -> MassAnswer:
     """One query, end to end, on the caller's current stream.  With a
     `stats` dict: H apply calls and band applies, eigensolver iterations,
     setup / solve / mass seconds and (per-kernel HIP-event timing on) the
     device kernel milliseconds are added to it.  rr_host: the Rayleigh-Ritz
-    eigenproblems on the host (LAPACK) instead of the device."""
+    eigenproblems on the host (LAPACK; False: on the device)."""
     t0 = time.perf_counter()
     wl = wl or build_workload(q)
     budget = budget if budget is not None else estimate_bytes(q, wl, prec)
@@ -187,10 +187,12 @@ def run_query(q: MassQuery, prec="mixed", device=0, wl=None, budget=None, stats=
             stats.setdefault(key, 0)
         stats["setup_s"] += time.perf_counter() - t0
     band = q.band if q.band >= 0 else q.nb // 2
-    # the small Rayleigh-Ritz eigenproblems in FP64 on the device (rocSOLVER
-    # through torch): on a 150-query burst with 8 slots 9.8 queries/s against
-    # 9.2 with host LAPACK (2 threads per slot), eigenvalues equal to 4e-14;
-    # rr_host=True keeps them on the host
+    # the small Rayleigh-Ritz eigenproblems (nb + buffer rows square, FP64):
+    # on the host by default (LAPACK, 2 threads per slot); rr_host=False runs
+    # them on the device (rocSOLVER through torch, eigenvalues equal to
+    # 4e-14).  Measured with 8 slots: 100 queries at 4/s p50 0.074-0.077 s,
+    # p99 0.48-0.51 s on the host against 0.105-0.131 s / 0.71-0.72 s on the
+    # device; at saturation (150 queries at once) 9.2-9.8 queries/s either way
     rr_eigh = eigensolver.eigh_host if rr_host else None
     inv, en, npws = [], [], []
     try:
@@ -298,7 +300,7 @@ def _set_sync_mode(device, sync):
         raise RuntimeError(f"hipSetDeviceFlags({sync}) failed with HIP error {rc}")
 
 
-def _worker_main(slot, device, prec, tasks, results, host_threads=2, sync="auto"):
+def _worker_main(slot, device, prec, tasks, results, host_threads=2, sync="auto", rr_host=True):
     """A serving slot: its own process, HIP context and stream; its host
     LAPACK work (the small Rayleigh-Ritz eigenproblems) uses host_threads
     threads so that the slots share the host cores; `sync` sets how its
@@ -322,7 +324,7 @@ def _worker_main(slot, device, prec, tasks, results, host_threads=2, sync="auto"
             return
         q, need, t_arr = item
         try:
-            ans = run_query(q, prec, device, budget=need)
+            ans = run_query(q, prec, device, budget=need, rr_host=rr_host)
             ans.latency = ans.finish - t_arr
         except Exception as e:                                   # surfaced in the answer
             now = time.perf_counter()
@@ -346,7 +348,7 @@ class MassServer:
     MAX_SLOTS = 15
 
     def __init__(self, device=0, hbm_budget=None, slots=4, prec="mixed", start_timeout=600.0,
-                 split_kpoints=True, host_threads=2, sync="auto"):
+                 split_kpoints=True, host_threads=2, sync="auto", rr_host=True):
         if not 1 <= slots <= self.MAX_SLOTS:
             raise ValueError(f"slots must be 1..{self.MAX_SLOTS}")
         if sync not in ("auto",) + tuple(_SYNC_FLAGS):
@@ -360,7 +362,7 @@ class MassServer:
         self._tasks = [ctx.Queue() for _ in range(slots)]
         self._results = ctx.Queue()
         self._procs = [ctx.Process(target=_worker_main,
-                                   args=(i, device, prec, self._tasks[i], self._results, host_threads, sync),
+                                   args=(i, device, prec, self._tasks[i], self._results, host_threads, sync, rr_host),
                                    daemon=True, name=f"mass-slot-{i}") for i in range(slots)]
         for p in self._procs:
             p.start()
@@ -527,12 +529,12 @@ class MassServer:
 
 
 def serve(queries, device=0, slots=4, prec="mixed", hbm_budget=None, realtime=True, timeout=None, progress=None,
-          split_kpoints=True, host_threads=2, sync="auto"):
+          split_kpoints=True, host_threads=2, sync="auto", rr_host=True):
     """Replay `queries` at their arrival times (realtime) or all at once;
     return (answers by qid, stats).  progress(done, total) is called about
     once a second while waiting."""
     srv = MassServer(device=device, hbm_budget=hbm_budget, slots=slots, prec=prec, split_kpoints=split_kpoints,
-                     host_threads=host_threads, sync=sync)
+                     host_threads=host_threads, sync=sync, rr_host=rr_host)
     try:
         t0 = time.perf_counter()
         for q in sorted(queries, key=lambda x: x.arrival):
