@@ -261,16 +261,21 @@ struct ProjBatch {
 };
 
 // Workgroup tile 64 x TN: TN = 64 (waves 2 x 2, each 2 x 2 blocks of
-// 16 x 16) or, for N <= 16 (one band per call, small eigensolver blocks),
-// TN = 16 (waves 4 x 1, one block each): a 64-wide tile there spends 64x
-// the MFMA work of the product and made the call MFMA-bound.
+// 16 x 16); for N <= 32 (the shrinking row blocks of a Chebyshev filter,
+// small eigensolver blocks) TN = 32 (waves 2 x 2, each 2 x 1 blocks); for
+// N <= 16 (one band per call) TN = 16 (waves 4 x 1, one block each): a
+// 64-wide tile there spends 64x the MFMA work of the product and made the
+// call MFMA-bound.
 template <int TN> struct GemmWaves {
-    static constexpr int WGC = TN == 64 ? 2 : 1;    // waves along N
+    static constexpr int WGC = TN == 16 ? 1 : 2;    // waves along N
     static constexpr int WGR = 4 / WGC;             // waves along M
     static constexpr int WR = 64 / (16 * WGR);      // 16-row blocks per wave
     static constexpr int WC = TN / (16 * WGC);      // 16-column blocks per wave
-    static_assert(TN == 64 || TN == 16, "tile width");
+    static_assert(TN == 64 || TN == 32 || TN == 16, "tile width");
 };
+
+// Tile width for an N-column product.
+inline int gemm_tile_width(int n) { return n <= 16 ? 16 : n <= 32 ? 32 : 64; }
 
 // Workgroups per CU the projection's register allocation must allow: FP32
 // keeps four (its LDS allows four) with the second register set of slabs in
@@ -656,9 +661,10 @@ __device__ __forceinline__ void apply_tile(const ApplySegK<T>& sg, int tm0, int 
     constexpr int NL = BK / RSTEP;                    // loads per lane, matrix and slab
     static_assert(NL * RSTEP == BK, "slab rows must split evenly over the loads");
     // B slabs are BK x TN: TN = 64 as A; TN = 16 unpadded (a 16-lane read
-    // cycle covers one whole 128- or 256-byte row), loaded by the first
-    // BK * TN / VEC lanes
-    constexpr int LDB = TN == 64 ? LDM : TN;
+    // cycle covers one whole 128- or 256-byte row); TN = 32 rows of 48 (the
+    // two 16-lane halves of an FP32 8-byte read cycle, k-rows kk and kk + 1,
+    // start 32 banks apart); loaded by the first BK * TN / VEC lanes
+    constexpr int LDB = TN == 64 ? LDM : TN == 32 ? 48 : TN;
     constexpr int CPRB = TN / VEC;
     constexpr int RSTEPB = 256 / CPRB;
     constexpr int NLB = (BK + RSTEPB - 1) / RSTEPB;
@@ -872,7 +878,7 @@ __global__ void __launch_bounds__(256, 2)
 k_gemm_apply(const ApplyBatch<T> bt, int N, int K, int accumulate) {
     using C = cx_t<T>;
     constexpr int BK = ApplyShape<T>::BK, LDM = ApplyShape<T>::LDM;
-    constexpr int LDB = TN == 64 ? LDM : TN;
+    constexpr int LDB = TN == 64 ? LDM : TN == 32 ? 48 : TN;
     __shared__ __align__(16) C sA[2 * BK * LDM];
     __shared__ __align__(16) C sB[2 * BK * LDB];
     const int ny = (N + TN - 1) / TN;
@@ -968,12 +974,18 @@ static hipError_t gemm_proj(Ctx& c, const ApplySeg* segs, int nseg, const cx_t<T
                              partial + (size_t)i * nsplit * Mtot * nb, (int64_t)npw, segs[i].ld, npw, kchunk,
                              A2 ? A2[i] : nullptr, A2 ? M1 : M};
     }
-    // nb <= 16: one 64 x 16 tile column (same grid as one 64 x 64 one)
+    // nb <= 32: one 64 x 16 or 64 x 32 tile column (same grid as one 64 x 64 one)
     dim3 grid(((M + 63) / 64) * ((nb + 63) / 64) * nsplit * nseg);
-    if (nb <= 16)
-        hipLaunchKernelGGL((k_gemm_proj<T, 16>), grid, dim3(256), 0, c.stream, bt, M, nb, nsplit, Mtot, row0);
-    else
-        hipLaunchKernelGGL((k_gemm_proj<T, 64>), grid, dim3(256), 0, c.stream, bt, M, nb, nsplit, Mtot, row0);
+    switch (gemm_tile_width(nb)) {
+        case 16:
+            hipLaunchKernelGGL((k_gemm_proj<T, 16>), grid, dim3(256), 0, c.stream, bt, M, nb, nsplit, Mtot, row0);
+            break;
+        case 32:
+            hipLaunchKernelGGL((k_gemm_proj<T, 32>), grid, dim3(256), 0, c.stream, bt, M, nb, nsplit, Mtot, row0);
+            break;
+        default:
+            hipLaunchKernelGGL((k_gemm_proj<T, 64>), grid, dim3(256), 0, c.stream, bt, M, nb, nsplit, Mtot, row0);
+    }
     return hipGetLastError();
 }
 
@@ -1008,11 +1020,19 @@ static hipError_t gemm_apply(Ctx& c, int nseg, const ApplySegK<T>* in, int nb, i
     // 16 takes the 64 x 16 tile (the same tile count as 64 x 64 there)
     const int a = acc ? 1 : 0;
     bt.full = tiles;
-    if (nb <= 16) {
-        if (bt.s[0].A2)
-            hipLaunchKernelGGL((k_gemm_apply<T, true, 16>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
-        else
-            hipLaunchKernelGGL((k_gemm_apply<T, false, 16>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
+    if (nb <= 32) {
+        // one 64 x 16 or 64 x 32 tile column: the same tile count as 64 x 64
+        if (nb <= 16) {
+            if (bt.s[0].A2)
+                hipLaunchKernelGGL((k_gemm_apply<T, true, 16>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
+            else
+                hipLaunchKernelGGL((k_gemm_apply<T, false, 16>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
+        } else {
+            if (bt.s[0].A2)
+                hipLaunchKernelGGL((k_gemm_apply<T, true, 32>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
+            else
+                hipLaunchKernelGGL((k_gemm_apply<T, false, 32>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
+        }
         return hipGetLastError();
     }
     // Workgroups on one CU share its MFMA pipes, so the kernel lasts as long
@@ -1165,10 +1185,11 @@ hipError_t launch_overlap(Ctx& c, int m, int n, int k, const void* A, int64_t ld
                          k, kchunk, nullptr, m};
     if (timed) timer_begin(c);
     dim3 grid(((m + 63) / 64) * ((n + 63) / 64) * nsplit);
-    if (n <= 16)
-        hipLaunchKernelGGL((k_gemm_proj<T, 16>), grid, dim3(256), 0, c.stream, bt, m, n, nsplit, m, 0);
-    else
-        hipLaunchKernelGGL((k_gemm_proj<T, 64>), grid, dim3(256), 0, c.stream, bt, m, n, nsplit, m, 0);
+    switch (gemm_tile_width(n)) {
+        case 16: hipLaunchKernelGGL((k_gemm_proj<T, 16>), grid, dim3(256), 0, c.stream, bt, m, n, nsplit, m, 0); break;
+        case 32: hipLaunchKernelGGL((k_gemm_proj<T, 32>), grid, dim3(256), 0, c.stream, bt, m, n, nsplit, m, 0); break;
+        default: hipLaunchKernelGGL((k_gemm_proj<T, 64>), grid, dim3(256), 0, c.stream, bt, m, n, nsplit, m, 0);
+    }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
         ReduceBatch<T> rb{};

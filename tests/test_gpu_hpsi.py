@@ -460,16 +460,17 @@ def test_large_grids(grid):
     assert e <= 1e-12 and e32 <= 2e-5 and de <= RY_TOL_HA
 
 
-@pytest.mark.parametrize("nb", [1, 2, 15, 16, 17])
+@pytest.mark.parametrize("nb", [1, 2, 15, 16, 17, 24, 32, 33])
 def test_narrow_band_counts(nb):
-    """Band counts at and around the 16-wide GEMM tiles (one band per call,
-    as the reference's callers apply H) on the 8-atom cell (64 projector
-    channels, npw in the thousands: the narrow tiles and the many-split
-    reduce both run): FP64 H|psi> and dH/dk at 1e-12 against the oracle,
-    mixed at 2e-5; the same bands through a 24-band call agree with the
-    narrow call to 1e-13 (FP64)."""
+    """Band counts at and around the 16- and 32-wide GEMM tiles (one band
+    per call, as the reference's callers apply H; the shrinking row blocks
+    of a Chebyshev filter) on the 8-atom cell (64 projector channels, npw in
+    the thousands: the narrow tiles and the many-split reduce both run):
+    FP64 H|psi> and dH/dk at 1e-12 against the oracle, mixed at 2e-5; the
+    same bands through a 40-band call (64-wide tiles) agree with the narrow
+    call to 1e-13 (FP64)."""
     torch = _torch()
-    wl = workload("si8_cubic", nb=24)
+    wl = workload("si8_cubic", nb=40)
     kp = wl.kpts[0]
     psi = wl.psi[0][:nb]
     ref = oracle.hpsi(wl.cell, kp, psi, nthreads=8)
