@@ -37,3 +37,19 @@ def test_footprint_grows_with_size():
     big = serving.MassQuery(qid=1, natom=16, ecut_ry=50.0, nb=128, kfracs=[np.zeros(3)], seed=1)
     es, eb = serving.estimate_bytes(small), serving.estimate_bytes(big)
     assert 0 < es < eb
+
+
+def test_server_options_are_checked_before_any_slot_starts():
+    import pytest
+    with pytest.raises(ValueError, match="slots"):
+        serving.MassServer(slots=0)
+    with pytest.raises(ValueError, match="sync"):
+        serving.MassServer(slots=1, sync="sleepy")
+    # "auto" leaves the HIP runtime's choice alone (no device call)
+    serving._set_sync_mode(0, "auto")
+
+
+def test_rayleigh_ritz_on_host_by_default():
+    import inspect
+    for fn in (serving.run_query, serving._worker_main, serving.serve, serving.MassServer.__init__):
+        assert inspect.signature(fn).parameters["rr_host"].default is True
