@@ -194,6 +194,16 @@ def main():
         roofline = {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
                     "frac": dom["frac"], "traffic": traffic, "kernel": dom["kernel"], "prec": "mixed",
                     "traffic_source": TRAFFIC_PROFILE if traffic is not None else None}
+        if dom["bound"] == "hbm":
+            # what a plain device copy sustains on this device in this run
+            # (bytes read + written per second), and the dominant kernel's
+            # measured traffic against it
+            copy_gbs = device_copy_gbs()
+            roofline["device_copy_gbs"] = copy_gbs
+            if traffic is not None:
+                rate = traffic / (dom["avg_us"] * 1e-6) / 1e9
+                roofline["traffic_gbs"] = rate
+                roofline["traffic_frac_of_copy"] = rate / copy_gbs
 
     # CPU baseline: the FP64 oracle (reference loop structure) on host cores
     cpu = None
@@ -309,6 +319,28 @@ def main():
         "setup_s": setup_s,
     }
     print(json.dumps(out), flush=True)
+
+
+def device_copy_gbs(mib=1024, reps=5):
+    """Bytes read + written per second by torch's device-to-device copy of
+    `mib` MiB (HIP events, best of `reps`): the rate a streaming kernel can
+    reach on this device."""
+    import torch
+    a = torch.ones(mib * (1 << 20) // 8, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * mib * (1 << 20) / (best * 1e-3) / 1e9
 
 
 def library_gemm_leg(npw, nb, nch, ours, reps=5):
