@@ -26,19 +26,21 @@ RY_PER_HA = 2.0
 TRAFFIC_PROFILE = "profiles/hpsi_si64_traffic.md"
 
 
-def measured_traffic(kernel, prec, pencil_bands=None):
+def measured_traffic(kernel, prec, shapes=None):
     """HBM bytes per launch of a kernel class at the 64-atom workload, from the
     rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE scaled by what they read
     for a plain 1 GiB device copy); for the x-pencil kernel the variant with
-    this run's bands per workgroup.  None if the summary is absent."""
+    this run's z width and bands per workgroup.  None if the summary is
+    absent."""
     path = os.path.join(ROOT, TRAFFIC_PROFILE)
     try:
         text = open(path).read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
         ent = json.loads(block)[kernel][prec]
-        if pencil_bands is not None:
+        if shapes is not None:
+            suffix = f", {shapes['pencil_z']}, {shapes['pencil_bands']}>"
             for name, t in ent.get("variants", {}).items():
-                if name.endswith(f", {pencil_bands}>"):
+                if name.endswith(suffix):
                     return t
         return ent["traffic_bytes"]
     except (OSError, IndexError, KeyError, ValueError):
@@ -132,7 +134,10 @@ def main():
 
     precs = (args.only,) if args.only else ("fp64", "mixed")
     # launch shapes picked by the autotuner on this cell, outside the timed region
-    tuned = {prec: dict(zip(("pencil_bands", "gemm_wgs"), H.autotune(0, nb, prec))) for prec in precs}
+    tuned = {}
+    for prec in precs:
+        H.autotune(0, nb, prec)
+        tuned[prec] = H.shapes(prec)
     res = {}
     for prec in precs:
         run(prec, args.warmup)
@@ -190,7 +195,7 @@ def main():
     roofline = None                       # needs the per-kernel timing steps (--timing-steps > 0)
     if dom is not None:
         traffic = measured_traffic(dom["kernel"], "mixed",
-                                   tuned.get("mixed", {}).get("pencil_bands") if dom["kernel"] == "xpencil_vloc" else None)
+                                   tuned.get("mixed") if dom["kernel"] == "xpencil_vloc" else None)
         roofline = {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
                     "frac": dom["frac"], "traffic": traffic, "kernel": dom["kernel"], "prec": "mixed",
                     "traffic_source": TRAFFIC_PROFILE if traffic is not None else None}

@@ -31,8 +31,14 @@ EXPORTS = (
     "effm_mem_info", "effm_cell_upload", "effm_kpt_upload", "effm_kpt_dim",
     "effm_kpt_release", "effm_hpsi", "effm_dhdk", "effm_hpsi_multi", "effm_dhdk_multi",
     "effm_hpsi_dhdk_multi", "effm_lincomb", "effm_rayleigh",
-    "effm_overlap", "effm_overlap_noalloc", "effm_kp_mass", "effm_autotune", "effm_set_timing", "effm_get_timing", "effm_reset_timing",
+    "effm_overlap", "effm_overlap_noalloc", "effm_kp_mass", "effm_autotune",
+    "effm_get_shapes", "effm_set_shapes", "effm_set_timing", "effm_get_timing", "effm_reset_timing",
 )
+
+
+class Shapes(C.Structure):
+    """effm_shapes: the autotuner's launch-shape knobs of one precision."""
+    _fields_ = [("pencil_bands", C.c_int), ("pencil_z", C.c_int), ("gemm_wgs", C.c_int), ("apply_tile", C.c_int)]
 
 
 class Species(C.Structure):
@@ -119,6 +125,8 @@ def load() -> C.CDLL:
     lib.effm_lincomb.argtypes = [vp, i64, C.c_double, vp, C.c_double, vp, C.c_double, vp, i32, i32]
     lib.effm_rayleigh.argtypes = [vp, i32, i32, vp, i64, vp, i64, i32, vp, i32]
     lib.effm_autotune.argtypes = [vp, i32, i32, i32, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.effm_get_shapes.argtypes = [vp, i32, C.POINTER(Shapes)]
+    lib.effm_set_shapes.argtypes = [vp, i32, C.POINTER(Shapes)]
     lib.effm_overlap.argtypes = [vp, i32, i32, i32, vp, i64, vp, i64, vp, i64, i32, i32]
     lib.effm_overlap_noalloc.argtypes = [vp, i32, i32, i32, vp, i64, vp, i64, vp, i64, i32, i32]
     lib.effm_kp_mass.argtypes = [vp, i32, vp, vp, i64, i32, vp, vp, C.c_double, vp, i32]

@@ -10,6 +10,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
+#include <iterator>
 #include <mutex>
 #include <new>
 
@@ -448,6 +450,9 @@ static int run_maybe_graph(Ctx& c, bool graph, std::vector<int64_t> key, F&& enq
 // ---------------------------------------------------------------------------
 //  autotuner: time each candidate launch shape on this cell / k-point
 // ---------------------------------------------------------------------------
+// Each knob is timed on the kernels it changes (per-kernel HIP events inside
+// an apply), not on the whole apply: the rest of the apply only adds noise
+// (whole-apply timing once picked a pencil shape that was slower per kernel).
 template <typename T>
 static int autotune(Ctx& c, KptDev& k, int nb, int* pencil_bands, int* gemm_wgs) {
     using C = cx_t<T>;
@@ -459,51 +464,69 @@ static int autotune(Ctx& c, KptDev& k, int nb, int* pencil_bands, int* gemm_wgs)
     HIP_TRY(hipMemsetAsync(c.stage_in.ptr, 0x3c, bytes, c.stream), "tuning fill");
     ApplySeg seg{&k, c.stage_in.ptr, k.npw, {c.stage_out0.ptr, nullptr, nullptr}, k.npw};
     Ctx::Tune& tn = c.tune[sizeof(T) == 8 ? 0 : 1];
+    // the caller's timing switch and accumulated timers are kept aside
+    timer_resolve(c);
     const bool timing = c.timing;
-    c.timing = false;
-    hipEvent_t ev[2];
-    HIP_TRY(hipEventCreate(&ev[0]), "tuning event");
-    if (hipEventCreate(&ev[1]) != hipSuccess) {
-        (void)hipEventDestroy(ev[0]);
-        return fail(c, kErrHip, "tuning event");
-    }
-    // best of three applies after one warm-up, for the current shape
-    auto measure = [&](float* best) -> int {
+    auto saved = std::move(c.timers);
+    c.timers.clear();
+    c.timing = true;
+    // best of three applies after one warm-up: the summed milliseconds of
+    // the named kernel classes (0 when none of them runs on this cell)
+    auto measure = [&](std::initializer_list<const char*> names, float* best) -> int {
         *best = 1e30f;
-        int r = run_hpsi<T>(c, &seg, 1, nb, false, false);
-        if (r) return r;
+        int r0 = run_hpsi<T>(c, &seg, 1, nb, false, false);   // allocation pass
+        if (r0) return r0;
         for (int rep = 0; rep < 4; ++rep) {
-            HIP_TRY(hipEventRecord(ev[0], c.stream), "tuning record");
-            if ((r = run_hpsi<T>(c, &seg, 1, nb, false, true))) return r;
-            HIP_TRY(hipEventRecord(ev[1], c.stream), "tuning record");
-            HIP_TRY(hipEventSynchronize(ev[1]), "tuning sync");
-            float ms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]), "tuning time");
-            if (rep > 0) *best = std::min(*best, ms);
+            c.timers.clear();
+            const int r = run_hpsi<T>(c, &seg, 1, nb, false, true);
+            timer_resolve(c);
+            if (r) return r;
+            double ms = 0.0;
+            for (const char* n : names) {
+                auto it = c.timers.find(n);
+                if (it != c.timers.end()) ms += it->second.ms;
+            }
+            if (rep > 0) *best = std::min(*best, (float)ms);
         }
         return kOk;
     };
+    // the candidate of `choices` (written to *knob) with the least time;
+    // the knob keeps its value when its kernels do not run
+    auto pick = [&](int* knob, const int* choices, int nchoice, std::initializer_list<const char*> names) -> int {
+        float best = 1e30f, t = 0.f;
+        int keep = *knob, r = kOk;
+        for (int i = 0; i < nchoice; ++i) {
+            *knob = choices[i];
+            if ((r = measure(names, &t))) break;
+            if (t <= 0.f) break;
+            if (t < best) best = t, keep = choices[i];
+        }
+        *knob = keep;
+        return r;
+    };
     rc = kOk;
-    float best = 1e30f, t = 0.f;
-    int pick = tn.pencil_bands;
-    for (int cand : kPencilBandChoices) {
-        tn.pencil_bands = cand;
-        if ((rc = measure(&t))) break;
-        if (t < best) best = t, pick = cand;
+    // x-pencil: band group and z width together (one kernel, 3 x 2 shapes)
+    if (rc == kOk) {
+        float best = 1e30f, t = 0.f;
+        int kb = tn.pencil_bands, kz = tn.pencil_z;
+        for (int bz : kPencilBandChoices) {
+            for (int z : kPencilZChoices) {
+                tn.pencil_bands = bz;
+                tn.pencil_z = z;
+                if ((rc = measure({"xpencil_vloc"}, &t))) break;
+                if (t > 0.f && t < best) best = t, kb = bz, kz = z;
+            }
+            if (rc) break;
+        }
+        tn.pencil_bands = kb;
+        tn.pencil_z = kz;
     }
-    tn.pencil_bands = pick;
-    best = 1e30f;
-    pick = tn.gemm_wgs;
-    for (int cand : kGemmWgChoices) {
-        if (rc) break;
-        tn.gemm_wgs = cand;
-        if ((rc = measure(&t))) break;
-        if (t < best) best = t, pick = cand;
-    }
-    tn.gemm_wgs = pick;
+    if (rc == kOk)
+        rc = pick(&tn.gemm_wgs, kGemmWgChoices, (int)std::size(kGemmWgChoices), {"proj_gemm", "proj_reduce"});
+    if (rc == kOk && nb > 32)
+        rc = pick(&tn.apply_tile, kApplyTileChoices, (int)std::size(kApplyTileChoices), {"proj_apply"});
     tn.tuned = rc == kOk;
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
+    c.timers = std::move(saved);
     c.timing = timing;
     ++c.epoch;    // captured graphs hold the old shapes: recapture
     if (pencil_bands) *pencil_bands = tn.pencil_bands;
@@ -1066,6 +1089,46 @@ int effm_autotune(effm_ctx* h, int ikpt, int nb, int prec, int* pencil_bands, in
     KptDev& k = c.kpts[ikpt];
     return prec == EFFM_PREC_FP64 ? autotune<double>(c, k, nb, pencil_bands, gemm_wgs)
                                   : autotune<float>(c, k, nb, pencil_bands, gemm_wgs);
+}
+
+static bool one_of(int v, const int* choices, int n) {
+    for (int i = 0; i < n; ++i)
+        if (choices[i] == v) return true;
+    return false;
+}
+
+int effm_get_shapes(const effm_ctx* h, int prec, effm_shapes* out) {
+    if (!h || !out) return kErrInvalid;
+    if (prec != EFFM_PREC_FP64 && prec != EFFM_PREC_MIXED) return kErrInvalid;
+    const Ctx& c = h->c;
+    const Ctx::Tune& tn = c.tune[prec == EFFM_PREC_FP64 ? 0 : 1];
+    out->pencil_bands = tn.pencil_bands;
+    out->pencil_z = prec == EFFM_PREC_FP64 ? pencil_z_in_effect<double>(c) : pencil_z_in_effect<float>(c);
+    out->gemm_wgs = tn.gemm_wgs;
+    out->apply_tile = tn.apply_tile;
+    return kOk;
+}
+
+int effm_set_shapes(effm_ctx* h, int prec, const effm_shapes* in) {
+    if (!h) return kErrInvalid;
+    Ctx& c = h->c;
+    if (!in) return fail(c, kErrInvalid, "shapes is NULL");
+    if (prec != EFFM_PREC_FP64 && prec != EFFM_PREC_MIXED) return fail(c, kErrInvalid, "unknown precision");
+    if (!one_of(in->pencil_bands, kPencilBandChoices, (int)std::size(kPencilBandChoices)))
+        return fail(c, kErrInvalid, "pencil_bands must be 4, 8 or 16");
+    if (in->pencil_z != 0 && !one_of(in->pencil_z, kPencilZChoices, (int)std::size(kPencilZChoices)))
+        return fail(c, kErrInvalid, "pencil_z must be 0, 16 or 18");
+    if (!one_of(in->gemm_wgs, kGemmWgChoices, (int)std::size(kGemmWgChoices)))
+        return fail(c, kErrInvalid, "gemm_wgs must be 512, 768, 1024, 1536 or 2048");
+    if (!one_of(in->apply_tile, kApplyTileChoices, (int)std::size(kApplyTileChoices)))
+        return fail(c, kErrInvalid, "apply_tile must be 64 or 32");
+    Ctx::Tune& tn = c.tune[prec == EFFM_PREC_FP64 ? 0 : 1];
+    tn.pencil_bands = in->pencil_bands;
+    tn.pencil_z = in->pencil_z;
+    tn.gemm_wgs = in->gemm_wgs;
+    tn.apply_tile = in->apply_tile;
+    ++c.epoch;    // captured graphs hold the old shapes: recapture
+    return kOk;
 }
 
 int effm_overlap(effm_ctx* h, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,

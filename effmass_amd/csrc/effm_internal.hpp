@@ -134,13 +134,17 @@ struct Ctx {
     // per kernel: resident workgroups per CU (dynamic LDS attribute set once)
     std::map<const void*, int> occupancy;
 
-    // launch shapes, per precision (0 = FP64, 1 = mixed): bands per x-pencil
-    // workgroup and the workgroup count the split-K projection aims for.
-    // Defaults are the measured best on the 90^3 / 64-atom cell;
-    // effm_autotune re-picks them by timing the candidates on this cell.
+    // launch shapes, per precision (0 = FP64, 1 = mixed): bands and z
+    // points per x-pencil workgroup (z 0 = the grid's default), the
+    // workgroup count the split-K projection aims for and the band-tile
+    // width of the apply GEMMs above 32 bands.  Defaults are the measured
+    // best on the 90^3 / 64-atom cell; effm_autotune re-picks them by timing
+    // the candidates on this cell.
     struct Tune {
         int pencil_bands = 8;
+        int pencil_z = 0;
         int gemm_wgs = 1024;
+        int apply_tile = 64;
         bool tuned = false;
     };
     Tune tune[2];
@@ -285,7 +289,12 @@ void prime_occupancy(Ctx& c);
 
 // candidate launch shapes effm_autotune times
 constexpr int kPencilBandChoices[] = {4, 8, 16};
+constexpr int kPencilZChoices[] = {16, 18};
 constexpr int kGemmWgChoices[] = {512, 768, 1024, 1536, 2048};
+constexpr int kApplyTileChoices[] = {64, 32};
+// z points per x-pencil workgroup in effect for precision T on this grid
+template <typename T>
+int pencil_z_in_effect(const Ctx& c);
 
 template <typename T>
 hipError_t launch_lincomb(Ctx& c, int64_t n, double a, void* z, double b, const void* y, double cc,

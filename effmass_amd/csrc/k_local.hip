@@ -822,8 +822,9 @@ k_xpencil_vloc_ct(const LocalBatch<T> bt, const T* __restrict__ vloc, int n2, in
     }
 }
 
-// z-chunk width of the pencil kernel (measured: FP32 prefers chunks of one
-// 128-byte line, FP64 on the 90-point grid five chunks of 18)
+// default z-chunk width of the pencil kernel, one of kPencilZChoices
+// (measured: FP32 prefers chunks of one 128-byte line, FP64 on the
+// 90-point grid five chunks of 18); effm_autotune times both widths
 template <typename T, int N> struct PencilShape {
     static constexpr int TZ = 16, W = 1;
     static constexpr bool VR = true, HOIST = true, BPF = false;
@@ -904,9 +905,8 @@ static hipError_t launch_planes_ct(Ctx& c, const LocalBatch<T>& bt, int nitems, 
     return hipGetLastError();
 }
 
-template <typename T, int N, int NBW>
+template <typename T, int N, int TZ, int NBW>
 static void launch_pencil_ct_nbw(Ctx& c, const LocalBatch<T>& bt, const T* vl, const cx_t<T>* tw1) {
-    constexpr int TZ = PencilShape<T, N>::TZ;
     constexpr int BLOCK = TZ * cmax(Plan2<N>::P1, Plan2<N>::P2);
     const int nzc = (c.n3 + TZ - 1) / TZ;
     const int nbg = (bt.nb + NBW - 1) / NBW;
@@ -914,15 +914,38 @@ static void launch_pencil_ct_nbw(Ctx& c, const LocalBatch<T>& bt, const T* vl, c
     hipLaunchKernelGGL((k_xpencil_vloc_ct<T, N, TZ, NBW>), gx, dim3(BLOCK), 0, c.stream, bt, vl, c.n2, c.n3, tw1);
 }
 
-// bands per workgroup: the tuned value (one of kPencilBandChoices)
-template <typename T, int N>
-static void launch_pencil_ct(Ctx& c, const LocalBatch<T>& bt, const T* vl, const cx_t<T>* tw1) {
+template <typename T, int N, int TZ>
+static void launch_pencil_ct_z(Ctx& c, const LocalBatch<T>& bt, const T* vl, const cx_t<T>* tw1) {
     switch (c.tuning<T>().pencil_bands) {
-        case 4: launch_pencil_ct_nbw<T, N, 4>(c, bt, vl, tw1); break;
-        case 16: launch_pencil_ct_nbw<T, N, 16>(c, bt, vl, tw1); break;
-        default: launch_pencil_ct_nbw<T, N, 8>(c, bt, vl, tw1); break;
+        case 4: launch_pencil_ct_nbw<T, N, TZ, 4>(c, bt, vl, tw1); break;
+        case 16: launch_pencil_ct_nbw<T, N, TZ, 16>(c, bt, vl, tw1); break;
+        default: launch_pencil_ct_nbw<T, N, TZ, 8>(c, bt, vl, tw1); break;
     }
 }
+
+// z points and bands per workgroup: the tuned values (kPencilZChoices,
+// kPencilBandChoices)
+template <typename T, int N>
+static void launch_pencil_ct(Ctx& c, const LocalBatch<T>& bt, const T* vl, const cx_t<T>* tw1) {
+    static_assert(PencilShape<T, N>::TZ == 16 || PencilShape<T, N>::TZ == 18, "a candidate width");
+    const int tz = c.tuning<T>().pencil_z ? c.tuning<T>().pencil_z : PencilShape<T, N>::TZ;
+    if (tz == 18) launch_pencil_ct_z<T, N, 18>(c, bt, vl, tw1);
+    else launch_pencil_ct_z<T, N, 16>(c, bt, vl, tw1);
+}
+
+template <typename T>
+int pencil_z_in_effect(const Ctx& c) {
+    if (c.tuning<T>().pencil_z) return c.tuning<T>().pencil_z;
+    switch (c.n1) {
+#define EFFM_PENCIL_Z_CASE(N) \
+    case N: return PencilShape<T, N>::TZ;
+        EFFM_CT_SIZES(EFFM_PENCIL_Z_CASE)
+#undef EFFM_PENCIL_Z_CASE
+        default: return 16;
+    }
+}
+template int pencil_z_in_effect<double>(const Ctx&);
+template int pencil_z_in_effect<float>(const Ctx&);
 
 template <typename T>
 static hipError_t planes_ct(Ctx& c, int n, const LocalBatch<T>& bt, int nitems, bool accumulate, bool g2r,

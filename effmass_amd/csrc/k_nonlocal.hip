@@ -1007,7 +1007,9 @@ template <typename T>
 static hipError_t gemm_apply(Ctx& c, int nseg, const ApplySegK<T>* in, int nb, int K, bool acc) {
     ApplyBatch<T> bt{};
     int tiles = 0;
-    const int ny = (nb + 63) / 64;
+    // above 32 bands: 64 x 64 tiles, or 64 x 32 ones (the tuned width)
+    const bool narrow = nb > 32 && c.tuning<T>().apply_tile == 32;
+    const int ny = narrow ? (nb + 31) / 32 : (nb + 63) / 64;
     for (int i = 0; i < nseg; ++i) {
         if (!in[i].out) continue;
         ApplySegK<T>& s = bt.s[bt.nseg++];
@@ -1033,6 +1035,13 @@ static hipError_t gemm_apply(Ctx& c, int nseg, const ApplySegK<T>* in, int nb, i
             else
                 hipLaunchKernelGGL((k_gemm_apply<T, false, 32>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
         }
+        return hipGetLastError();
+    }
+    if (narrow) {
+        if (bt.s[0].A2)
+            hipLaunchKernelGGL((k_gemm_apply<T, true, 32>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
+        else
+            hipLaunchKernelGGL((k_gemm_apply<T, false, 32>), dim3(tiles), dim3(256), 0, c.stream, bt, nb, K, a);
         return hipGetLastError();
     }
     // Workgroups on one CU share its MFMA pipes, so the kernel lasts as long

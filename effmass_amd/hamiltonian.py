@@ -353,6 +353,27 @@ class Hamiltonian:
                    self._ctx, "effm_autotune")
         return pb.value, gw.value
 
+    SHAPE_KEYS = ("pencil_bands", "pencil_z", "gemm_wgs", "apply_tile")
+
+    def shapes(self, prec: str = "fp64") -> dict:
+        """The launch shapes in effect for `prec` (effm_get_shapes)."""
+        s = _lib.Shapes()
+        _lib.check(self._lib.effm_get_shapes(self._ctx, PREC[prec], C.byref(s)), self._ctx, "effm_get_shapes")
+        return {k: getattr(s, k) for k in self.SHAPE_KEYS}
+
+    def set_shapes(self, prec: str = "fp64", **shapes):
+        """Set some of the launch shapes of `prec`; the others keep the
+        values in effect.  A value outside its candidate list raises
+        (effm_set_shapes)."""
+        unknown = set(shapes) - set(self.SHAPE_KEYS)
+        if unknown:
+            raise TypeError(f"unknown launch shapes: {sorted(unknown)}")
+        cur = self.shapes(prec)
+        cur.update(shapes)
+        self._sync_stream()
+        _lib.check(self._lib.effm_set_shapes(self._ctx, PREC[prec], C.byref(_lib.Shapes(**cur))), self._ctx,
+                   "effm_set_shapes")
+
     def set_timing(self, on: bool):
         self._lib.effm_set_timing(self._ctx, 1 if on else 0)
 

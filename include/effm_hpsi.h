@@ -214,13 +214,35 @@ int effm_overlap(effm_ctx* ctx, int m, int n, int k, const void* A, int64_t lda,
 int effm_kp_mass(effm_ctx* ctx, int nb, const double* E, const void* const* V, int64_t ldv, int nsel,
                  const int* bands, const double* d2h, double degen_tol, double* inv_mass, int flags);
 
-/* Autotuner: times the candidate launch shapes (bands per x-pencil
- * workgroup: 4 / 8 / 16; split-K workgroup target of the projection GEMM:
- * 512 ... 2048) with nb bands at k-point ikpt in precision prec, on
- * scratch buffers, and keeps the fastest for every later apply in that
- * precision.  pencil_bands / gemm_wgs (may be NULL) receive the choice.
- * No reference counterpart. */
+/* Launch shapes of one precision, the knobs the autotuner sets.  Every
+ * choice gives the same results (the FFT knobs bit for bit, the GEMM ones
+ * up to the order of the split-K partial sums); only the time differs. */
+typedef struct effm_shapes {
+    int pencil_bands; /* bands per x-pencil FFT workgroup: 4, 8, 16 */
+    int pencil_z;     /* z points per x-pencil FFT workgroup: 16, 18; 0 = the
+                       * grid's default (18 for the FP64 90-point x axis, else
+                       * 16); effm_get_shapes reports the value in effect */
+    int gemm_wgs;     /* split-K workgroup target of the projection GEMM:
+                       * 512, 768, 1024, 1536, 2048 */
+    int apply_tile;   /* band-tile width of the apply GEMMs above 32 bands:
+                       * 64 (64 x 64 tiles), 32 (64 x 32 tiles) */
+} effm_shapes;
+
+/* Autotuner: times the candidate launch shapes with nb bands at k-point
+ * ikpt in precision prec, on scratch buffers, each knob on the kernel it
+ * changes (x-pencil bands and z points on the x-pencil pass, the split-K
+ * target on the projection GEMM and its reduction, the tile width on the
+ * apply GEMM), and keeps the fastest for every later apply in that
+ * precision.  pencil_bands / gemm_wgs (may be NULL) receive those two
+ * choices; effm_get_shapes returns all of them.  No reference counterpart. */
 int effm_autotune(effm_ctx* ctx, int ikpt, int nb, int prec, int* pencil_bands, int* gemm_wgs);
+/* The launch shapes in effect for prec (tuned, set, or the defaults
+ * 8 / 0 / 1024 / 64).  No reference counterpart. */
+int effm_get_shapes(const effm_ctx* ctx, int prec, effm_shapes* out);
+/* Sets the launch shapes of prec (e.g. ones tuned before on the same cell);
+ * a value outside its candidate list is EFFM_ERR_INVALID and changes
+ * nothing.  No reference counterpart. */
+int effm_set_shapes(effm_ctx* ctx, int prec, const effm_shapes* in);
 
 /* Per-kernel timing with HIP events on the context's stream (0 = off). */
 int effm_set_timing(effm_ctx* ctx, int enable);

@@ -245,8 +245,11 @@ def test_autotune_choices_keep_parity():
     H.load_kpoint(0, wl.kpts[0])
     for prec in ("fp64", "mixed"):
         pb, gw = H.autotune(0, 24, prec)
-        log(f"{prec}: pencil bands {pb}, split-K workgroups {gw}")
+        sh = H.shapes(prec)
+        log(f"{prec}: {sh}")
         assert pb in (4, 8, 16) and gw in (512, 768, 1024, 1536, 2048)
+        assert (sh["pencil_bands"], sh["gemm_wgs"]) == (pb, gw)
+        assert sh["pencil_z"] in (16, 18) and sh["apply_tile"] == 64     # 24 bands: no tile-width choice
     psi = torch.from_numpy(wl.psi[0][:24]).cuda()
     h = torch.zeros_like(psi)
     H.at_k(0)(psi, h, overwrite=True)
@@ -350,4 +353,69 @@ def test_multi_split_plane_grid_and_graph():
     torch.cuda.synchronize()
     for ik in range(3):
         assert torch.equal(outs[ik], eager[ik])
+    H.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_every_launch_shape_gives_the_same_result(prec):
+    """Every candidate launch shape gives the default shape's results on the
+    64-atom cell (90^3, 512 projector channels) with 40 bands (two 32-wide
+    apply tiles, the second one partial): the x-pencil band group 4 / 8 / 16
+    and z width 16 / 18 bit for bit (every band and z column is transformed
+    on its own), the apply tile width 64 / 32 bit for bit (each output keeps
+    its k order) for H|psi> and dH/dk, the split-K targets to 1e-13 (FP64) /
+    1e-6 (mixed) relative per band (partial sums grouped differently); the
+    default shape equals the oracle (FP64 1e-12, mixed 2e-5).  Values outside
+    the candidate lists are refused and change nothing."""
+    torch = _torch()
+    nb = 40
+    wl = workload("si64_cubic", nb=nb)
+    kp = wl.kpts[0]
+    dt = torch.complex128 if prec == "fp64" else torch.complex64
+    H = Hamiltonian(wl.cell)
+    H.load_kpoint(0, kp)
+    p = torch.from_numpy(wl.psi[0][:nb]).to(dt).cuda()
+
+    def run():
+        h = torch.zeros_like(p)
+        H.at_k(0)(p, h, prec=prec)
+        g = [torch.zeros_like(p) for _ in range(3)]
+        H.gradient().at_k(0)(p, GradHPsi(*g), prec=prec)
+        torch.cuda.synchronize()
+        return h.cpu().numpy(), [x.cpu().numpy() for x in g]
+
+    default = H.shapes(prec)
+    assert default == {"pencil_bands": 8, "pencil_z": 18 if prec == "fp64" else 16, "gemm_wgs": 1024,
+                       "apply_tile": 64}
+    h0, g0 = run()
+    ref = oracle.hpsi(wl.cell, kp, wl.psi[0][:4], nthreads=8)
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    assert band_rel(h0[:4], ref) <= tol
+    for pb in (4, 8, 16):
+        for pz in (16, 18):
+            H.set_shapes(prec, pencil_bands=pb, pencil_z=pz)
+            h, _ = run()
+            log(f"{prec} pencil {pb} bands x {pz} z: max |diff| {np.abs(h - h0).max():.1e}")
+            assert np.array_equal(h, h0)
+    H.set_shapes(prec, **default)
+    H.set_shapes(prec, apply_tile=32)
+    assert H.shapes(prec)["apply_tile"] == 32
+    h, g = run()
+    log(f"{prec} apply tile 32: max |diff| H {np.abs(h - h0).max():.1e}, "
+        f"dH/dk {max(np.abs(g[a] - g0[a]).max() for a in range(3)):.1e}")
+    assert np.array_equal(h, h0) and all(np.array_equal(g[a], g0[a]) for a in range(3))
+    H.set_shapes(prec, apply_tile=64)
+    stol = 1e-13 if prec == "fp64" else 1e-6
+    for w in (512, 768, 1536, 2048):
+        H.set_shapes(prec, gemm_wgs=w)
+        h, g = run()
+        e = max([band_rel(h, h0)] + [band_rel(g[a], g0[a]) for a in range(3)])
+        log(f"{prec} split-K target {w}: {e:.1e}")
+        assert e <= stol
+    H.set_shapes(prec, **default)
+    from effmass_amd._lib import EffmInvalidArgument
+    for bad in ({"pencil_bands": 32}, {"pencil_z": 20}, {"gemm_wgs": 1000}, {"apply_tile": 16}):
+        with pytest.raises(EffmInvalidArgument):
+            H.set_shapes(prec, **bad)
+        assert H.shapes(prec) == default
     H.close()
