@@ -29,3 +29,15 @@ def test_status_codes_match_header():
                  "EFFM_F_OVERWRITE", "EFFM_F_HOST"):
         m = re.search(rf"#define {name}\s+(-?\w+)", hdr)
         assert m and int(m.group(1), 0) == getattr(_lib, name), name
+
+
+def test_struct_fields_match_header():
+    """The ctypes mirrors of the header's structs list the same fields in
+    the same order (effm_shapes is passed by pointer to effm_get_shapes /
+    effm_set_shapes)."""
+    hdr = open(os.path.join(ROOT, "include", "effm_hpsi.h")).read()
+    body = re.search(r"typedef struct effm_shapes \{(.*?)\} effm_shapes;", hdr, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = re.findall(r"int\s+(\w+)\s*;", body)
+    assert fields == [f for f, _ in _lib.Shapes._fields_]
+    assert ctypes.sizeof(_lib.Shapes) == 4 * len(fields)
