@@ -419,3 +419,38 @@ def test_every_launch_shape_gives_the_same_result(prec):
             H.set_shapes(prec, **bad)
         assert H.shapes(prec) == default
     H.close()
+
+
+@pytest.mark.parametrize("grid", [(28, 30, 32), (120, 120, 120)])
+def test_autotune_on_generic_and_split_grids(grid):
+    """The autotuner on cells outside the compile-time plans: the runtime-plan
+    kernels (28 x 30 x 32) and the split-plane kernels (120^3, whose FP64
+    plane exceeds the LDS), with 40 bands so that the apply tile width is
+    tuned as well.  After tuning, H|psi> equals the default shapes' result
+    (FP64 1e-13, mixed 1e-6 relative per band: the split-K partial sums may
+    be grouped differently) and the oracle on two bands (FP64 1e-12, mixed
+    2e-5)."""
+    torch = _torch()
+    A, tau = synth.diamond_cubic(10.26)
+    nb = 40
+    wl = synth.custom(A, tau, grid, 25.0, [np.array([0.1, -0.05, 0.2])], nb, seed=7 + sum(grid))
+    kp = wl.kpts[0]
+    assert kp.npw >= nb
+    ref = oracle.hpsi(wl.cell, kp, wl.psi[0][:2], nthreads=8)
+    H = Hamiltonian(wl.cell)
+    H.load_kpoint(0, kp)
+    for prec, dt, tol, stol in (("fp64", torch.complex128, 1e-12, 1e-13), ("mixed", torch.complex64, 2e-5, 1e-6)):
+        p = torch.from_numpy(wl.psi[0]).to(dt).cuda()
+        h0 = torch.zeros_like(p)
+        H.at_k(0)(p, h0, prec=prec, overwrite=True)
+        H.autotune(0, nb, prec)
+        sh = H.shapes(prec)
+        h = torch.zeros_like(p)
+        H.at_k(0)(p, h, prec=prec, overwrite=True)
+        torch.cuda.synchronize()
+        got, base = h.cpu().numpy(), h0.cpu().numpy()
+        e_shape = band_rel(got, base)
+        e = band_rel(got[:2].astype(np.complex128), ref)
+        log(f"grid {grid} {prec} npw {kp.npw}: tuned {sh}; vs default shapes {e_shape:.1e}, vs oracle {e:.2e}")
+        assert e_shape <= stol and e <= tol
+    H.close()
