@@ -535,3 +535,39 @@ def test_apply_gemm_quarter_tiles():
         log(f"quarter tiles {prec}: batch vs per-band {e_call:.2e}, tail vs oracle {e_ref:.2e}")
         assert e_call <= tol_call and e_ref <= tol_ref
     H.close()
+
+
+def test_thousand_bands():
+    """1000 bands of the 8-atom cell (npw 6451, 48^3) in one call: the
+    widest launches in this suite (16 band tiles of 64 per GEMM tile row,
+    125 x-pencil band groups of 8, 1000 x 23 plane items).  FP64 H|psi> and
+    dH/dk against the oracle on the first, middle and last bands (1e-12),
+    mixed at 2e-5; the 1000-band call equals two 500-band calls to 1e-13
+    (FP64, relative per band: the split-K partial sums may be grouped
+    differently) and 1e-6 (mixed)."""
+    torch = _torch()
+    nb = 1000
+    wl = workload("si8_cubic", nb=nb)
+    kp = wl.kpts[0]
+    pick = [0, 499, 500, 999]
+    ref = oracle.hpsi(wl.cell, kp, wl.psi[0][pick], nthreads=8)
+    gref = oracle.dhdk(wl.cell, kp, wl.psi[0][[0, 999]])
+    H = Hamiltonian(wl.cell)
+    H.load_kpoint(0, kp)
+    for prec, dt, tol, stol in (("fp64", torch.complex128, 1e-12, 1e-13), ("mixed", torch.complex64, 2e-5, 1e-6)):
+        p = torch.from_numpy(wl.psi[0]).to(dt).cuda()
+        h = torch.zeros_like(p)
+        H.at_k(0)(p, h, prec=prec)
+        halves = [torch.zeros_like(p[:500]) for _ in range(2)]
+        H.at_k(0)(p[:500].contiguous(), halves[0], prec=prec)
+        H.at_k(0)(p[500:].contiguous(), halves[1], prec=prec)
+        g = [torch.zeros_like(p) for _ in range(3)]
+        H.gradient().at_k(0)(p, GradHPsi(*g), prec=prec)
+        torch.cuda.synchronize()
+        hn = h.cpu().numpy()
+        e = band_rel(hn[pick].astype(np.complex128), ref)
+        es = band_rel(np.concatenate([x.cpu().numpy() for x in halves]), hn)
+        eg = max(band_rel(g[a].cpu().numpy()[[0, 999]].astype(np.complex128), gref[a]) for a in range(3))
+        log(f"1000 bands {prec}: hpsi vs oracle {e:.2e}, dH/dk vs oracle {eg:.2e}, vs two 500-band calls {es:.1e}")
+        assert e <= tol and eg <= tol and es <= stol
+    H.close()
