@@ -82,6 +82,29 @@ def velocity_matrices(H: Hamiltonian, ikpt: int, X: torch.Tensor, prec="fp64"):
     return [H.overlap(X64, d.to(torch.complex128)) for d in D]
 
 
+def velocity_columns(H: Hamiltonian, ikpt: int, X: torch.Tensor, bands, prec="fp64"):
+    """The entries of the velocity matrices the k.p sum of `bands` reads:
+    V[a][m][n] = <m|dH/dk_a|n> for n in bands (dH/dk applied to those
+    states only, FP64 overlaps with all rows of X) and the rows V[a][n][m] =
+    conj(V[a][m][n]) (dH/dk_a is Hermitian); complex128 [nb, nb] device
+    tensors, zero elsewhere.  For a few bands of many states this applies
+    dH/dk to len(bands) states instead of nb."""
+    nb = X.shape[0]
+    idx = torch.as_tensor([int(n) for n in bands], dtype=torch.long, device=X.device)
+    Xs = X[idx].to(torch.complex128 if prec == "fp64" else torch.complex64).contiguous()
+    D = [torch.zeros_like(Xs) for _ in range(3)]
+    H.gradient().at_k(ikpt)(Xs, GradHPsi(*D), prec=prec, overwrite=True)
+    X64 = X.to(torch.complex128).contiguous()
+    V = []
+    for d in D:
+        cols = H.overlap(X64, d.to(torch.complex128))          # [nb, len(bands)]: <m|dH/dk_a|n>
+        Va = torch.zeros((nb, nb), dtype=torch.complex128, device=X.device)
+        Va[:, idx] = cols
+        Va[idx, :] = cols.conj().T
+        V.append(Va)
+    return V
+
+
 def d2h_expect(H: Hamiltonian, kp: synth.KPoint, A, X: torch.Tensor, bands, prec="fp64",
                delta=1e-3, slot=None):
     """<n| d^2H / dk_a dk_b |n> for the bands (rows of X) by central
@@ -112,7 +135,8 @@ def effective_mass(H: Hamiltonian, ikpt: int, kp: synth.KPoint, A, X: torch.Tens
     """(1/m*) for `bands` from the eigenpairs (X rows, E) of H at ikpt."""
     Ed = (E if torch.is_tensor(E) else torch.as_tensor(np.asarray(E, dtype=np.float64))).to(
         device=X.device, dtype=torch.float64)
-    V = velocity_matrices(H, ikpt, X, prec)
+    # only the rows / columns of the asked bands enter the sum
+    V = velocity_columns(H, ikpt, X, bands, prec)
     d2 = d2h_expect(H, kp, A, X, bands, prec, delta)
     inv = H.kp_mass(Ed, V, bands, d2, degen_tol).cpu().numpy()
     En = Ed.cpu().numpy()

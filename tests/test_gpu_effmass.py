@@ -118,3 +118,21 @@ def test_device_kp_sum_equals_host_sum():
         assert np.abs(got[i] - ref).max() <= 1e-13 * np.abs(ref).max(), n
         if n in (10, 11):
             assert excl == [10, 11]
+
+
+def test_velocity_columns_equal_full_matrices():
+    """effective_mass reads only the rows and columns of the asked bands:
+    velocity_columns (dH/dk on those states only, rows as conjugate columns)
+    equals the full velocity matrices there to 1e-12 of their largest entry
+    (FP64, 200 states, bands 4 and 5)."""
+    wl, H, X, E, m, res = solve("fp64")
+    full = effmass.velocity_matrices(H, 0, X, "fp64")
+    cols = effmass.velocity_columns(H, 0, X, BANDS, "fp64")
+    for a in range(3):
+        F, Cm = full[a].cpu().numpy(), cols[a].cpu().numpy()
+        scale = np.abs(F).max()
+        for n in BANDS:
+            ec = np.abs(Cm[:, n] - F[:, n]).max() / scale
+            er = np.abs(Cm[n, :] - F[n, :]).max() / scale
+            print(f"dir {a} band {n}: column {ec:.1e}, row {er:.1e}", flush=True)
+            assert ec <= 1e-12 and er <= 1e-12
