@@ -22,11 +22,15 @@ EFFM_PLAN2(18, 6, 3)
 EFFM_PLAN2(20, 5, 4)
 EFFM_PLAN2(24, 6, 4)
 EFFM_PLAN2(25, 5, 5)
+EFFM_PLAN2(27, 9, 3)
+EFFM_PLAN2(32, 8, 4)
 EFFM_PLAN2(36, 6, 6)
 EFFM_PLAN2(40, 8, 5)
 EFFM_PLAN2(45, 9, 5)
 EFFM_PLAN2(48, 8, 6)
+EFFM_PLAN2(80, 10, 8)
 EFFM_PLAN2(90, 10, 9)
+EFFM_PLAN2(96, 12, 8)
 #undef EFFM_PLAN2
 
 // Prime-factor axes: with gcd(P1, P2) = 1 the two stages need no twiddle
@@ -47,8 +51,10 @@ template <int N> struct Pfa {
 };
 
 // The axis lengths above (the 64-atom cell's 90, the 8-atom 48, and the
-// grids of 2-64 atom cells at 30-50 Ry); X-macro for dispatch.
-#define EFFM_CT_SIZES(X) X(15) X(16) X(18) X(20) X(24) X(25) X(36) X(40) X(45) X(48) X(90)
+// density-sphere grids of 2-64 atom cells at 30-50 Ry: 27, 32, 36, 40, 45,
+// 48, 80, 90, 96); X-macro for dispatch.
+#define EFFM_CT_SIZES(X) \
+    X(15) X(16) X(18) X(20) X(24) X(25) X(27) X(32) X(36) X(40) X(45) X(48) X(80) X(90) X(96)
 
 // One radix-P stage (combining blocks of L) over NL lines of an N-point axis;
 // element e of line l at l*LS + e*ES.  Lines vary fastest across threads when

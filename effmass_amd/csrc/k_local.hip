@@ -374,6 +374,18 @@ struct PlaneTune<float> {
     static constexpr bool G2R_EARLY = true;
 };
 
+// Workgroup size of the square-plane kernels.  The 96-point planes (12 x 8)
+// have 1152 butterflies per stage: at the 1024-thread (FP64) / 512-thread
+// (FP32) default two or three column butterflies per thread spill (FP64
+// 128 VGPRs: 68 / 216 bytes per lane in G2R / R2G) or leave one workgroup
+// per CU (FP32 166 VGPRs), so they take 576 / 384 threads (exactly two /
+// three butterflies each; 170 VGPRs allowed, two FP32 workgroups per CU).
+template <typename T, int N>
+constexpr int plane_block_of() {
+    if constexpr (N == 96) return sizeof(T) == 8 ? 576 : 384;
+    return cmin(plane_block<N>(), PlaneTune<T>::BLOCK_CAP);
+}
+
 // Waits until every vector memory access this wave issued has completed
 // (s_waitcnt vmcnt(0), other counters unconstrained).  A real waitcnt
 // instruction: the compiler's own wait insertion counts it.
@@ -884,7 +896,7 @@ static hipError_t persistent_grid(Ctx& c, const void* fn, size_t lds, int block,
 template <typename T, int N>
 static hipError_t launch_planes_ct(Ctx& c, const LocalBatch<T>& bt, int nitems, bool accumulate,
                                    bool g2r, const cx_t<T>* tw) {
-    constexpr int BLOCK = cmin(plane_block<N>(), PlaneTune<T>::BLOCK_CAP);
+    constexpr int BLOCK = plane_block_of<T, N>();
     const size_t lds = sizeof(cx_t<T>) * ((size_t)N * N + N) + sizeof(int) * N;
     hipError_t e;
     int grid = 0;

@@ -76,10 +76,25 @@ def query_cell(natom: int):
     return np.diag(a * np.array(rep, dtype=float)), np.array(pos)
 
 
+# Axis lengths with compile-time two-stage FFT plans in the engine (the
+# EFFM_CT_SIZES list of effmass_amd/csrc/fft_ct.hpp).
+CT_FFT_SIZES = (15, 16, 18, 20, 24, 25, 27, 32, 36, 40, 45, 48, 80, 90, 96)
+
+
 def grid_for(A, ecut_ry):
-    """FFT grid holding the density sphere: n >= 2 G_max |a_i| / 2 pi + 1."""
+    """FFT grid holding the density sphere, as PWmat's grids do (the
+    reference's Si8 fixture: 36^3 at 25 Ry, a = 10.26 bohr): the wave
+    functions fill |K| <= G_max = sqrt(ecut_ry) (bohr^-1), their products
+    with V_loc reach 2 G_max, so n_i > 2 (2 G_max) |a_i| / 2 pi, i.e.
+    n_i >= ceil(4 G_max |a_i| / 2 pi) + 1 with no wrap-around of V psi.
+    Each axis takes the smallest length with a compile-time FFT plan that
+    holds it (else the smallest 2,3,5-smooth length)."""
     gmax = math.sqrt(ecut_ry)
-    return tuple(smooth_size(int(math.ceil(2 * gmax * np.linalg.norm(a) / (2 * math.pi))) + 1) for a in A)
+    out = []
+    for a in A:
+        need = int(math.ceil(4 * gmax * np.linalg.norm(a) / (2 * math.pi))) + 1
+        out.append(next((m for m in CT_FFT_SIZES if m >= need), None) or smooth_size(need))
+    return tuple(out)
 
 
 @dataclass
@@ -124,11 +139,9 @@ def make_stream(n, rate=50.0, seed=2024, nb_range=(16, 256), sizes=SIZES, ecuts=
 
 def build_workload(q: MassQuery, order: str = "plane"):
     """The query's cell, grid and k-points.  The sphere is stored plane by
-    plane (synth.g_sphere order "plane"): on these grids, which the sphere
-    fills densely, the |K|^2 order scattered every plane's reads and writes
-    over the whole band (64 atoms at 50 Ry, 320 bands: mixed H apply
-    1.84 -> 1.48 ms, the gather pass 478 -> 196 us); the masses do not
-    depend on the order."""
+    plane (synth.g_sphere order "plane"), so that each plane's scatter and
+    gather touch one contiguous run of every band rather than lines all
+    over it as in the |K|^2 order; the masses do not depend on the order."""
     A, tau = query_cell(q.natom)
     n = grid_for(A, q.ecut_ry)
     return synth.custom(A, tau, n, q.ecut_ry, q.kfracs, q.nb, q.seed, with_psi=False, order=order)

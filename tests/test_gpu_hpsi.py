@@ -294,11 +294,13 @@ def test_multi_k_ragged():
 @pytest.mark.parametrize("grid", [(28, 30, 32), (42, 45, 40), (24, 20, 36), (48, 49, 50),
                                   (15, 15, 15), (16, 16, 16), (18, 18, 18), (20, 20, 20), (25, 25, 25),
                                   (40, 24, 24), (45, 45, 45), (48, 25, 25), (48, 45, 45), (16, 40, 40),
-                                  (22, 26, 33), (26, 22, 39)])
+                                  (22, 26, 33), (26, 22, 39),
+                                  (27, 32, 32), (32, 27, 27), (96, 80, 80), (80, 96, 96)])
 def test_fft_radix_plans(grid):
     """Grids whose axes use radices 2..12 (incl. 7) in the runtime LDS plan,
     and every compile-time two-stage plan (square planes and pencils of
-    15, 16, 18, 20, 24, 25, 40, 45, 48): prime-factor order where the two
+    15, 16, 18, 20, 24, 25, 27, 32, 40, 45, 48, 80, 96; 36 and 90 in the
+    tests above): prime-factor order where the two
     radices are coprime (15, 20, 40, 45), Cooley-Tukey otherwise, and the
     two mixed between the pencil and the plane axes ((40, 24, 24): prime-
     factor pencil, Cooley-Tukey planes; (48, 45, 45), (16, 40, 40): the

@@ -13,12 +13,36 @@ def test_cells_and_grids():
         for ecut in serving.ECUTS:
             g = serving.grid_for(A, ecut)
             for m, a in zip(g, A):
-                assert m >= 2 * np.sqrt(ecut) * np.linalg.norm(a) / (2 * np.pi) + 1
+                # the density sphere (radius 2 sqrt(ecut)) without wrap-around
+                assert m >= 4 * np.sqrt(ecut) * np.linalg.norm(a) / (2 * np.pi) + 1
+                assert m in serving.CT_FFT_SIZES
                 k = m
                 for p in (2, 3, 5):
                     while k % p == 0:
                         k //= p
                 assert k == 1
+
+
+def test_grid_rule_matches_the_reference_fixture():
+    """The rule gives the grid of the reference's Si8 PWmat data (IN.VR on
+    36^3 at ecut 25 Ry, a = 10.26 bohr) and the 90^3 / 48^3 grids of the
+    64-atom (40 Ry) and 8-atom (50 Ry) cells."""
+    a = 10.26314072
+    assert serving.grid_for(np.diag([a] * 3), 25.0) == (36, 36, 36)
+    assert serving.grid_for(np.diag([2 * a] * 3), 40.0) == (90, 90, 90)
+    assert serving.grid_for(np.diag([a] * 3), 50.0) == (48, 48, 48)
+
+
+def test_ct_sizes_match_the_engine():
+    """CT_FFT_SIZES lists exactly the engine's compile-time plan lengths."""
+    import pathlib
+    import re
+    src = (pathlib.Path(serving.__file__).parent / "csrc" / "fft_ct.hpp").read_text()
+    plans = {int(m) for m in re.findall(r"^EFFM_PLAN2\((\d+),", src, re.M)}
+    body = src[src.index("#define EFFM_CT_SIZES"):]
+    body = body[:body.index("\n\n")]
+    listed = {int(m) for m in re.findall(r"X\((\d+)\)", body)}
+    assert set(serving.CT_FFT_SIZES) == plans == listed
 
 
 def test_stream_is_seeded_poisson():
