@@ -226,18 +226,17 @@ def make_vloc(A, n, tau, rng: SplitMix64, amp_ry=-2.0, sigma=1.0,
     g1, g2, g3 = np.meshgrid(*f, indexing="ij")
     G = np.stack([g1, g2, g3], axis=-1) @ B
     G2 = np.einsum("...i,...i->...", G, G)
-    sf = np.zeros(n, dtype=np.complex128)
-    for t in tau:
-        p1 = np.exp(-2j * np.pi * f[0] * t[0])
-        p2 = np.exp(-2j * np.pi * f[1] * t[1])
-        p3 = np.exp(-2j * np.pi * f[2] * t[2])
-        sf += p1[:, None, None] * p2[None, :, None] * p3[None, None, :]
+    # structure factor sum_t p1_t(i) p2_t(j) p3_t(k) as one matrix product
+    # over the atoms: [(i, j), t] x [t, k]
+    tau = np.asarray(tau, dtype=np.float64).reshape(-1, 3)
+    p1, p2, p3 = (np.exp(-2j * np.pi * np.outer(tau[:, a], f[a])) for a in range(3))
+    sf = ((p1[:, :, None] * p2[:, None, :]).reshape(len(tau), n1 * n2).T @ p3).reshape(n)
     vg = (amp_ry / RY_PER_HA) * (2 * np.pi * sigma ** 2) ** 1.5 / omega * np.exp(-0.5 * sigma ** 2 * G2) * sf
-    v = np.fft.ifftn(vg).real * (n1 * n2 * n3)
     noise = rng.normal(n1 * n2 * n3).reshape(n) * (noise_ry / RY_PER_HA)
     nk = np.fft.fftn(noise)
     nk[G2 >= gcut_noise ** 2] = 0.0
-    v = v + np.fft.ifftn(nk).real
+    # the Gaussians' and the filtered noise's spectra back in one transform
+    v = np.fft.ifftn(vg * (n1 * n2 * n3) + nk).real
     return np.ascontiguousarray(v.ravel())
 
 
