@@ -506,11 +506,23 @@ __global__ void k_reduce_scale(const ReduceBatch<T> bt, int per_seg, int nsplit,
     const int64_t idx = (int64_t)(blockIdx.x - seg * per_seg) * blockDim.x + threadIdx.x;
     const int64_t tot = (int64_t)Mtot * N;
     if (idx >= tot) return;
-    const cx_t<T>* __restrict__ part = bt.part[seg];
+    const cx_t<T>* __restrict__ part = bt.part[seg] + idx;
     cx_t<T>* __restrict__ out = bt.out[seg];
     double re = 0.0, im = 0.0;
-    for (int s = 0; s < nsplit; ++s) {
-        cx_t<T> v = part[(int64_t)s * tot + idx];
+    // eight splits' loads in flight at a time, added in split order
+    int s = 0;
+    for (; s + 8 <= nsplit; s += 8) {
+        cx_t<T> v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = part[(int64_t)(s + u) * tot];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            re += (double)v[u].x;
+            im += (double)v[u].y;
+        }
+    }
+    for (; s < nsplit; ++s) {
+        const cx_t<T> v = part[(int64_t)s * tot];
         re += (double)v.x;
         im += (double)v.y;
     }
@@ -561,7 +573,9 @@ template <typename T>
 static hipError_t launch_reduce(Ctx& c, const ReduceBatch<T>& bt, int nseg, int nsplit, int Mtot, int N,
                                 const double* lam, int nch, int64_t ldo) {
     const int64_t tot = (int64_t)Mtot * N;
-    if (tot < 32768 && nsplit >= 16) {
+    // many splits and too few outputs for one 256-output workgroup per CU:
+    // 32 lanes per output
+    if (nsplit >= 16 && (tot < 32768 || tot < (int64_t)c.ncu * 256)) {
         const int per = (int)((tot + 7) / 8);
         hipLaunchKernelGGL((k_reduce_scale_wide<T>), dim3((unsigned)(per * nseg)), dim3(256), 0, c.stream, bt, per,
                            nsplit, Mtot, N, lam, nch, ldo);
