@@ -82,6 +82,10 @@ def main():
                     help="queries of a second stream at --serve-light-rate, below the server's "
                          "capacity, for the unqueued latency (0 = skip)")
     ap.add_argument("--serve-light-rate", type=float, default=4.0)
+    ap.add_argument("--serve-mid", type=int, default=200,
+                    help="queries of a third stream at --serve-mid-rate, near the server's capacity "
+                         "(0 = skip)")
+    ap.add_argument("--serve-mid-rate", type=float, default=8.0)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
                     help="time one precision only (profiling runs)")
@@ -258,7 +262,7 @@ def main():
 
     # serving: Poisson(50/s) effective-mass queries over 2-64 atom cells
     serving_stats = None
-    if args.serve_queries > 0 or args.serve_light > 0:
+    if args.serve_queries > 0 or args.serve_light > 0 or args.serve_mid > 0:
         from effmass_amd import serving
         torch.cuda.empty_cache()
         serving_stats = {"workload": ("Poisson arrivals; cells of 2/8/16/32/64 atoms at 135 bohr^3/atom, "
@@ -269,7 +273,8 @@ def main():
                                       "arrival to answer"),
                          "slots": args.serve_slots}
         for key, n, rate, seed in (("rate_50", args.serve_queries, 50.0, 2024),
-                                   ("below_capacity", args.serve_light, args.serve_light_rate, 2025)):
+                                   ("below_capacity", args.serve_light, args.serve_light_rate, 2025),
+                                   ("near_capacity", args.serve_mid, args.serve_mid_rate, 2026)):
             if n <= 0:
                 continue
             qs = serving.make_stream(n, rate=rate, seed=seed)
@@ -557,11 +562,11 @@ def facade_leg(workloads):
 
 
 def _latency_summary(st):
-    """p50/p99 of the 50/s stream (and of the below-capacity stream)."""
+    """p50/p99 of the 50/s stream (and of the below- and near-capacity streams)."""
     if not st:
         return None
     out = {}
-    for key in ("rate_50", "below_capacity"):
+    for key in ("rate_50", "below_capacity", "near_capacity"):
         if key in st:
             out[key] = {k: st[key][k] for k in ("p50_s", "p99_s", "queries_per_s", "queries", "rate_per_s")}
     return out
