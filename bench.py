@@ -87,6 +87,8 @@ def main():
                          "(0 = skip)")
     ap.add_argument("--serve-mid-rate", type=float, default=8.0)
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--kp-mass", type=int, default=1,
+                    help="1: the 8-atom cell's k.p masses in FP64 and mixed (kp_mass); 0 = skip")
     ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
                     help="time one precision only (profiling runs)")
     ap.add_argument("--multik-steps", type=int, default=3,
@@ -257,6 +259,8 @@ def main():
     facade = facade_leg([("si2_fcc", synth.build("si2_fcc")), ("si64_cubic", wl)]) if args.facade else None
     # 32 k-points of the 64-atom cell in batched launches, graph replays
     multik = multik_leg(args.multik_steps) if args.multik_steps > 0 else None
+    # k.p effective masses of the 8-atom cell, mixed vs FP64
+    kpm = kp_mass_leg() if args.kp_mass else None
     # the vendor library on GEMMs of the same shapes, for reference
     libgemm = library_gemm_leg(npw, nb, cell.nchannels(), {k["kernel"] + "/" + k["prec"]: k["avg_us"] for k in kernels})
 
@@ -322,6 +326,7 @@ def main():
         "kernels": kernels,
         "tuned_launch_shapes": tuned,
         "multi_k": multik,
+        "kp_mass": kpm,
         "library_gemm": libgemm,
         "facade": facade,
         "m_star_latency": _latency_summary(serving_stats),
@@ -485,6 +490,63 @@ def multik_leg(steps, warmup=1):
         torch.cuda.empty_cache()
     H.close()
     return res
+
+
+def kp_mass_leg():
+    """The 8-atom cubic cell (a = 10.26 bohr, ecut 50 Ry, 48^3, npw 6451) at
+    Gamma: the 200 lowest states by the Chebyshev-filtered subspace iteration
+    (plane-wave start, 48 buffer rows; residual 1e-8 Ha in FP64, 2e-5 mixed),
+    then 1/m* of bands 4 and 5 from the k.p sum over the 200 states (FP64
+    sums and denominators in both).  Reported: the tensors, the error of the
+    mixed path per tensor element against the FP64 path (whose masses equal
+    the oracle's on the same eigenpairs to 1e-9, tests/test_gpu_effmass.py),
+    and the device seconds of each part."""
+    import torch
+    from effmass_amd import effmass, eigensolver, synth
+    from effmass_amd.hamiltonian import Hamiltonian
+
+    wl = synth.build("si8_cubic", nb=2)
+    cell, kp = wl.cell, wl.kpts[0]
+    bands, nb, rows = [4, 5], 200, 248
+    out = {"workload": "si8_cubic: 8-atom cubic Si, a=10.26 bohr, ecut 50 Ry, npw %d, FFT 48^3, Gamma; "
+                       "200 states, 1/m* of bands 4, 5 by the k.p sum over them" % kp.npw}
+    res = {}
+    for prec in ("fp64", "mixed"):
+        H = Hamiltonian(cell)
+        H.load_kpoint(0, kp)
+        kin = torch.from_numpy(kp.kin).cuda()
+        op = H.at_k(0)
+        wdt = torch.complex128 if prec == "fp64" else torch.complex64
+
+        def apply(V, HV):
+            op(V, HV, prec=prec, overwrite=True)
+        # untimed warm-up of the same calls on a small block (library
+        # kernels, solver and allocator first calls)
+        Xw = eigensolver.plane_wave_start(apply, kin, 24, work_dtype=wdt, eigh=eigensolver.eigh_host)
+        eigensolver.chfsi(apply, Xw, kin, nconv=8, tol=1e-3, maxiter=2, work_dtype=wdt, gram=H.overlap,
+                          degree=8, eigh=eigensolver.eigh_host, lincomb=H.lincomb, buffer_degree="wanted")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        X0 = eigensolver.plane_wave_start(apply, kin, rows, work_dtype=wdt, eigh=eigensolver.eigh_host)
+        r = eigensolver.chfsi(apply, X0, kin, nconv=nb, tol=1e-8 if prec == "fp64" else 2e-5, maxiter=300,
+                              work_dtype=wdt, gram=H.overlap, degree=30, eigh=eigensolver.eigh_host,
+                              lincomb=H.lincomb, buffer_degree="wanted")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        m = effmass.effective_mass(H, 0, kp, cell.A, r.X[:nb].contiguous(), r.E[:nb], bands, prec=prec)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        H.close()
+        res[prec] = m
+        out[prec] = {"solve_s": t1 - t0, "kp_s": t2 - t1, "iters": r.iters, "converged": r.converged,
+                     "inv_mass": m.inv_mass.tolist(), "masses": [list(map(float, v)) for v in m.masses()]}
+    d = np.abs(res["mixed"].inv_mass - res["fp64"].inv_mass)
+    out["mixed_vs_fp64_abs_err_inv_mass"] = d.tolist()
+    out["mixed_vs_fp64_max_rel_err_inv_mass"] = float(d.max() / np.abs(res["fp64"].inv_mass).max())
+    mr = [np.abs(np.asarray(a) - np.asarray(b)) / np.abs(np.asarray(b))
+          for a, b in zip(res["mixed"].masses(), res["fp64"].masses())]
+    out["mixed_vs_fp64_max_rel_err_m_star"] = float(max(x.max() for x in mr))
+    return out
 
 
 def _cpu_split(oracle, cell, kp, psi, per_band_s):
