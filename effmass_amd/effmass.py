@@ -31,6 +31,7 @@ import torch
 
 from . import synth
 from .hamiltonian import GradHPsi, Hamiltonian
+from .hamiltonian import shifted_kpoint as _shifted_kpoint
 
 
 @dataclass
@@ -48,10 +49,7 @@ class MassResult:
 
 def shifted_kpoint(kp: synth.KPoint, A: np.ndarray, dk_cart) -> synth.KPoint:
     """Same G set at k + dk (Cartesian): K + dk, |K|^2/2, k_frac + dk.A^T/2pi."""
-    dk = np.asarray(dk_cart, dtype=np.float64)
-    K = kp.K + dk[None, :]
-    return synth.KPoint(kfrac=kp.kfrac + dk @ A.T / (2.0 * np.pi), K=np.ascontiguousarray(K),
-                        kin=0.5 * np.einsum("ij,ij->i", K, K), gidx=kp.gidx)
+    return _shifted_kpoint(kp, A, dk_cart)
 
 
 def kp_sum(E, V, d2h, n, degen_tol=1e-6):

@@ -90,6 +90,26 @@ class GradHPsi:
     z: Any = None
 
 
+@dataclass
+class HessHPsi:
+    """Output container of d2H/dk_a dk_b |psi> (hamiltonian.cppm:84-91): the
+    six unique components; None = skip."""
+    xx: Any = None
+    yy: Any = None
+    zz: Any = None
+    xy: Any = None
+    xz: Any = None
+    yz: Any = None
+
+
+def shifted_kpoint(kp: SynthKPoint, A, dk_cart) -> SynthKPoint:
+    """The same G set at k + dk (Cartesian): K + dk, |K|^2/2, k_frac + dk.A^T/2pi."""
+    dk = np.asarray(dk_cart, dtype=np.float64)
+    K = kp.K + dk[None, :]
+    return SynthKPoint(kfrac=kp.kfrac + dk @ np.asarray(A, dtype=np.float64).T / (2.0 * np.pi),
+                       K=np.ascontiguousarray(K), kin=0.5 * np.einsum("ij,ij->i", K, K), gidx=kp.gidx)
+
+
 class Hamiltonian:
     """Owns the device-resident cell and k-points (hamiltonian.cppm:115)."""
 
@@ -103,6 +123,7 @@ class Hamiltonian:
         self.cell = cell
         self._upload_cell(cell)
         self._npw = {}
+        self._kpts = {}          # host copies of the loaded k-points (the Hessian's shifted copies)
 
     # -- lifetime --------------------------------------------------------
     def close(self):
@@ -174,10 +195,12 @@ class Hamiltonian:
                                              gi.ctypes.data, kf.ctypes.data), self._ctx,
                    "effm_kpt_upload")
         self._npw[ikpt] = kp.npw
+        self._kpts[ikpt] = kp
 
     def release_kpoint(self, ikpt: int):
         _lib.check(self._lib.effm_kpt_release(self._ctx, ikpt), self._ctx, "effm_kpt_release")
         self._npw.pop(ikpt, None)
+        self._kpts.pop(ikpt, None)
 
     def mem_info(self):
         u, b = C.c_uint64(), C.c_uint64()
@@ -190,6 +213,9 @@ class Hamiltonian:
 
     def gradient(self) -> "Gradient":
         return Gradient(self)
+
+    def hessian(self, delta: float = 1e-3) -> "Hessian":
+        return Hessian(self, delta)
 
     def rayleigh(self, psi, hpsi, prec: str = "fp64"):
         """FP64 E_b = Re<psi_b|hpsi_b>/<psi_b|psi_b> (test_hpsi_eigen.cpp:49-56)."""
@@ -472,4 +498,106 @@ class GradientCallable:
             self._h._sync_stream()
         _lib.check(self._h._lib.effm_dhdk(self._h._ctx, self._ikpt, nb, pp, ld, ptrs[0], ptrs[1], ptrs[2],
                                           ldo, p, flags), self._h._ctx, "dH/dk|psi>")
+        return out
+
+
+class Hessian:
+    def __init__(self, parent: Hamiltonian, delta: float = 1e-3):
+        self._h = parent
+        self._delta = delta
+
+    def at_k(self, ikpt: int) -> "HessianCallable":
+        return HessianCallable(self._h, ikpt, self._delta)
+
+
+class HessianCallable:
+    """d2H/dk_a dk_b |psi> at fixed k, the operator the reference declares
+    (hamiltonian.cppm:267-306) and leaves empty (hamiltonian_hessian.cpp:48-50).
+    d2H/dk_a dk_b = delta_ab (kinetic) + d2V_NL/dk_a dk_b; here the whole of
+    it is the central difference over k_b of the analytic dH/dk_a at
+    k +- delta e_b (Cartesian, bohr^-1) on the same G set, which is exact for
+    the kinetic part (dH/dk_a = K_a there) and O(delta^2) for the nonlocal
+    part; xy, xz, yz take the difference of dH/dk_x, dH/dk_x, dH/dk_y along
+    y, z, z.  The shifted k-points go through one scratch slot past the
+    highest loaded one, released afterwards.  Results accumulate into the
+    outputs (overwrite=True replaces them); a None component is skipped.
+    psi / outputs: device tensors or host arrays, as for Callable; the
+    difference is taken in FP64."""
+
+    # shift direction b -> the components it gives (name, a)
+    _PLAN = ((0, (("xx", 0),)), (1, (("xy", 0), ("yy", 1))), (2, (("xz", 0), ("yz", 1), ("zz", 2))))
+
+    def __init__(self, parent: Hamiltonian, ikpt: int, delta: float):
+        self._h = parent
+        self._ikpt = ikpt
+        self._delta = float(delta)
+        self.dim()
+
+    def dim(self) -> int:
+        return _lib.check(self._h._lib.effm_kpt_dim(self._h._ctx, self._ikpt), self._h._ctx,
+                          "Hessian::Callable::dim")
+
+    def set_ikpt(self, ikpt: int):
+        self._ikpt = ikpt
+        self.dim()
+
+    def __call__(self, psi, out: HessHPsi, prec: str = "fp64", overwrite: bool = False,
+                 n_bands: Optional[int] = None):
+        import torch
+        p = PREC[prec]
+        dev = self._h.device
+        pi = _arr_info(psi, p, dev)
+        names = [n for n in ("xx", "yy", "zz", "xy", "xz", "yz") if getattr(out, n) is not None]
+        infos = []
+        for n in names:
+            oi = _arr_info(getattr(out, n), p, dev)
+            if oi[3] != pi[3]:
+                raise TypeError("psi and outputs must both be host or both be device arrays")
+            infos.append(oi)
+        if not names:
+            return out
+        nb = _bands(n_bands, self.dim(), pi, infos)
+        kp = self._h._kpts.get(self._ikpt)
+        if kp is None:
+            raise _lib.EffmOutOfRange(f"k-point {self._ikpt} was not loaded through this Hamiltonian")
+        npw = kp.npw
+        cdt = torch.complex128 if prec == "fp64" else torch.complex64
+        x = torch.as_tensor(psi, device=f"cuda:{dev}").reshape(-1, psi.shape[-1])[:nb, :npw]
+        x = x.to(cdt).contiguous()
+        slot = max(self._h._npw.keys()) + 1
+        res = {}
+        try:
+            for b, comps in self._PLAN:
+                want = [(n, a) for n, a in comps if n in names]
+                if not want:
+                    continue
+                D = {}
+                for sgn in (+1.0, -1.0):
+                    dk = np.zeros(3)
+                    dk[b] = sgn * self._delta
+                    self._h.load_kpoint(slot, shifted_kpoint(kp, self._h.cell.A, dk))
+                    g = [torch.zeros_like(x) if any(a == d for _, a in want) else None for d in range(3)]
+                    self._h.gradient().at_k(slot)(x, GradHPsi(*g), prec=prec, overwrite=True)
+                    D[sgn] = g
+                for n, a in want:
+                    res[n] = (D[1.0][a].to(torch.complex128) - D[-1.0][a].to(torch.complex128)) / (2 * self._delta)
+        finally:
+            if slot in self._h._npw:
+                self._h.release_kpoint(slot)
+        for n in names:
+            o = getattr(out, n)
+            v = res[n]
+            if _is_torch(o):
+                tgt = o.reshape(-1, o.shape[-1])[:nb, :npw]
+                if overwrite:
+                    tgt.copy_(v.to(o.dtype))
+                else:
+                    tgt.add_(v.to(o.dtype))
+            else:
+                tgt = o.reshape(-1, o.shape[-1])[:nb, :npw]
+                vh = v.cpu().numpy().astype(o.dtype)
+                if overwrite:
+                    tgt[...] = vh
+                else:
+                    tgt += vh
         return out

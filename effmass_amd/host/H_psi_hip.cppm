@@ -19,6 +19,7 @@ using effm::CellData;
 using effm::GradHPsi;
 using effm::GradHPsi32;
 using effm::Hamiltonian;
+using effm::HessHPsi;
 using effm::KPointData;
 using effm::Precision;
 using effm::SpeciesData;
@@ -27,3 +28,4 @@ using effm::throw_status;
 
 export using effm::GradHPsi;
 export using effm::Hamiltonian;
+export using effm::HessHPsi;

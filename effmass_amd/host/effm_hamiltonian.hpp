@@ -13,6 +13,8 @@
 //   op(n_bands, psi, hpsi);               // batched form, band-major
 //   auto g = h.gradient().at_k(ikpt);
 //   g(psi, effm::GradHPsi{dx, dy, dz});   // empty span = skip direction
+//   auto hs = h.hessian().at_k(ikpt);
+//   hs(psi, effm::HessHPsi{xx, yy, zz, xy, xz, yz});   // d2H/dk dk |psi>
 //
 // Spans are host memory, complex<double> (the reference's FP64 path) or
 // complex<float> (the mixed path: FP32 FFT / GEMM, FP64 D contraction); the
@@ -25,8 +27,10 @@
 // std::runtime_error for NOMEM.
 #pragma once
 
+#include <algorithm>
 #include <complex>
 #include <cstdint>
+#include <map>
 #include <span>
 #include <stdexcept>
 #include <string>
@@ -47,6 +51,17 @@ struct GradHPsi32 {
     std::span<std::complex<float>> x;
     std::span<std::complex<float>> y;
     std::span<std::complex<float>> z;
+};
+
+// d2H/dk_a dk_b |psi> outputs (reference: hamiltonian.cppm:84-91); an empty
+// span skips that component
+struct HessHPsi {
+    std::span<std::complex<double>> xx;
+    std::span<std::complex<double>> yy;
+    std::span<std::complex<double>> zz;
+    std::span<std::complex<double>> xy;
+    std::span<std::complex<double>> xz;
+    std::span<std::complex<double>> yz;
 };
 
 enum class Precision : int { FP64 = EFFM_PREC_FP64, Mixed = EFFM_PREC_MIXED };
@@ -101,6 +116,7 @@ public:
                                   s.lambda.data(), s.nq, s.dq, s.qmax, s.beta_tab.data(),
                                   s.dbeta_tab.empty() ? nullptr : s.dbeta_tab.data()};
         }
+        for (int i = 0; i < 9; ++i) lattice_[i] = cell.lattice[i];
         effm_cell c{};
         c.n1 = cell.n1; c.n2 = cell.n2; c.n3 = cell.n3;
         for (int i = 0; i < 9; ++i) c.lattice[i] = cell.lattice[i];
@@ -125,6 +141,7 @@ public:
     void loadKPoint(int ikpt, const KPointData& k) {
         throw_status(effm_kpt_upload(ctx_, ikpt, k.npw(), k.K.data(), k.kinetic.data(), k.gidx.data(), k.kfrac),
                      ctx_, "loadKPoint");
+        kpts_[ikpt] = k;          // host copy: the Hessian's shifted k-points are built from it
     }
 
     effm_ctx* ctx() const { return ctx_; }
@@ -233,8 +250,118 @@ public:
     };
     Gradient gradient() const { return Gradient(ctx_); }
 
+    // -- 3. Hessian: d2H/dk_a dk_b |psi> (hamiltonian.cppm:267-306) ---------
+    // The reference declares this operator and leaves its body empty
+    // (hamiltonian_hessian.cpp:48-50).  Here d2H/dk_a dk_b = delta_ab +
+    // d2V_NL/dk_a dk_b is the central difference over k_b of the analytic
+    // dH/dk_a at k +- delta e_b (Cartesian, bohr^-1) on the same G set: exact
+    // for the kinetic part (dH/dk_a = K_a there), O(delta^2) for the nonlocal
+    // part; xy, xz, yz difference dH/dk_x, dH/dk_x, dH/dk_y along y, z, z.
+    // The shifted k-points use one scratch k-point index past the highest
+    // one loaded, released afterwards.  Results accumulate into the spans.
+    class Hessian {
+    public:
+        class Callable {
+        public:
+            Callable() = default;
+            void operator()(std::span<const std::complex<double>> psi, HessHPsi out) const {
+                apply(1, psi, out);
+            }
+            // batched bands (band-major, dim() apart)
+            void operator()(int n_bands, std::span<const std::complex<double>> psi, HessHPsi out) const {
+                apply(n_bands, psi, out);
+            }
+            int dim() const {
+                int n = effm_kpt_dim(h_->ctx_, ikpt_);
+                throw_status(n, h_->ctx_, "Hessian::dim");
+                return n;
+            }
+            void set_ikpt(int ikpt) { ikpt_ = ikpt; (void)dim(); }
+
+        private:
+            friend class Hessian;
+            Callable(const Hamiltonian* h, int ikpt, double delta) : h_(h), ikpt_(ikpt), delta_(delta) { (void)dim(); }
+            void apply(int n_bands, std::span<const std::complex<double>> psi, HessHPsi out) const {
+                using cd = std::complex<double>;
+                const int ng = dim();
+                const size_t need = static_cast<size_t>(n_bands) * ng;
+                if (n_bands < 0 || psi.size() < need)
+                    throw std::invalid_argument("Hessian::Callable: psi span shorter than n_bands * dim()");
+                std::span<cd>* comp[6] = {&out.xx, &out.yy, &out.zz, &out.xy, &out.xz, &out.yz};
+                for (auto* c : comp)
+                    if (!c->empty() && c->size() < need)
+                        throw std::invalid_argument("HessHPsi span shorter than n_bands * dim()");
+                auto it = h_->kpts_.find(ikpt_);
+                if (it == h_->kpts_.end())
+                    throw std::out_of_range("Hessian: k-point " + std::to_string(ikpt_) +
+                                            " was not loaded through loadKPoint");
+                const KPointData& kp = it->second;
+                const int slot = h_->kpts_.rbegin()->first + 1;
+                // shift direction b -> (component, a): xx; xy, yy; xz, yz, zz
+                struct Part { int comp, a; };
+                const std::vector<Part> plan[3] = {{{0, 0}}, {{3, 0}, {1, 1}}, {{4, 0}, {5, 1}, {2, 2}}};
+                std::vector<cd> d[2][3];
+                struct Release {
+                    effm_ctx* c; int s; bool on = false;
+                    ~Release() { if (on) (void)effm_kpt_release(c, s); }
+                } rel{h_->ctx_, slot};
+                for (int b = 0; b < 3; ++b) {
+                    bool any = false;
+                    bool dir[3] = {false, false, false};
+                    for (const Part& q : plan[b])
+                        if (!comp[q.comp]->empty()) any = dir[q.a] = true;
+                    if (!any) continue;
+                    for (int sg = 0; sg < 2; ++sg) {
+                        const double dk = (sg == 0 ? 1.0 : -1.0) * delta_;
+                        KPointData sk = kp;
+                        for (int g = 0; g < sk.npw(); ++g) {
+                            double* K = &sk.K[3 * static_cast<size_t>(g)];
+                            K[b] += dk;
+                            sk.kinetic[g] = 0.5 * (K[0] * K[0] + K[1] * K[1] + K[2] * K[2]);
+                        }
+                        // k_frac + dk . A^T / 2 pi (rows of A are the lattice vectors)
+                        for (int i = 0; i < 3; ++i) sk.kfrac[i] += dk * h_->lattice_[3 * i + b] / (2.0 * kPi);
+                        throw_status(effm_kpt_upload(h_->ctx_, slot, sk.npw(), sk.K.data(), sk.kinetic.data(),
+                                                     sk.gidx.data(), sk.kfrac),
+                                     h_->ctx_, "Hessian: shifted k-point");
+                        rel.on = true;
+                        void* ptr[3] = {nullptr, nullptr, nullptr};
+                        for (int a = 0; a < 3; ++a)
+                            if (dir[a]) {
+                                d[sg][a].assign(need, cd(0.0));
+                                ptr[a] = d[sg][a].data();
+                            }
+                        throw_status(effm_dhdk(h_->ctx_, slot, n_bands, psi.data(), ng, ptr[0], ptr[1], ptr[2], ng,
+                                               EFFM_PREC_FP64, EFFM_F_HOST | EFFM_F_OVERWRITE),
+                                     h_->ctx_, "Hessian: dH/dk at k +- delta");
+                    }
+                    const double s = 1.0 / (2.0 * delta_);
+                    for (const Part& q : plan[b]) {
+                        std::span<cd>& o = *comp[q.comp];
+                        if (o.empty()) continue;
+                        for (size_t i = 0; i < need; ++i) o[i] += (d[0][q.a][i] - d[1][q.a][i]) * s;
+                    }
+                }
+            }
+            static constexpr double kPi = 3.14159265358979323846;
+            const Hamiltonian* h_ = nullptr;
+            int ikpt_ = 0;
+            double delta_ = 1e-3;
+        };
+        Callable at_k(int ikpt) const { return Callable(h_, ikpt, delta_); }
+
+    private:
+        friend class Hamiltonian;
+        Hessian(const Hamiltonian* h, double delta) : h_(h), delta_(delta) {}
+        const Hamiltonian* h_ = nullptr;
+        double delta_ = 1e-3;
+    };
+    Hessian hessian(double delta = 1e-3) const { return Hessian(this, delta); }
+
 private:
     effm_ctx* ctx_ = nullptr;
+    double lattice_[9] = {0};
+    std::map<int, KPointData> kpts_;
 };
 
 }  // namespace effm

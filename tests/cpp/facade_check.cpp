@@ -2,7 +2,8 @@
 // reference's tests drive H_psi::Hamiltonian: at_k(ikpt)(psi, hpsi) with
 // accumulate semantics, gradient().at_k(ikpt)(psi, GradHPsi{...}) with a
 // skipped direction, and the exception classes of the error paths; then the
-// mixed-precision host spans and the device-pointer forms.
+// mixed-precision host spans, the device-pointer forms and
+// hessian().at_k(ikpt)(psi, HessHPsi{...}).
 // Closed-form case: simple cubic cell, constant V, one s projector
 // beta(q) = exp(-q^2/4) on one atom, so H psi is known analytically.
 // Built twice: with #include of the facade (C++20), and with
@@ -156,6 +157,38 @@ int main() {
     (void)hipFree(dgx);
     std::printf("facade device H|psi> max err %.3e, dH/dk %.3e\n", derr, dgerr);
     if (derr > 2e-9 * ref || dgerr > 2e-8) return 7;
+    // Hessian, closed form: d2H/dk_a dk_b psi = delta_ab psi + lam (<d_ab p|psi> p
+    // + <d_a p|psi> d_b p + <d_b p|psi> d_a p + <p|psi> d_ab p), with p ~ f(|K|),
+    // f = exp(-|K|^2/4): d_a f = -(K_a/2) f, d_ab f = (K_a K_b/4 - delta_ab/2) f.
+    // Outputs start at (0.25 + 0.5i) (accumulate); xz is skipped.
+    {
+        const int ca[6] = {0, 1, 2, 0, 0, 1}, cb[6] = {0, 1, 2, 1, 2, 2};
+        const cd base(0.25, 0.5);
+        std::vector<cd> hs[6];
+        for (auto& v : hs) v.assign(npw, base);
+        h.hessian().at_k(0)(psi, effm::HessHPsi{hs[0], hs[1], hs[2], hs[3], {}, hs[5]});
+        double herr = 0, href = 0;
+        std::vector<cd> d2p(npw);
+        for (int c = 0; c < 6; ++c) {
+            const int a = ca[c], b = cb[c];
+            cd in2 = 0;
+            for (int g = 0; g < npw; ++g) {
+                const double* K = &k.K[3 * g];
+                d2p[g] = (K[a] * K[b] / 4.0 - (a == b ? 0.5 : 0.0)) * p[g];
+                in2 += std::conj(d2p[g]) * psi[g];
+            }
+            for (int g = 0; g < npw; ++g) {
+                cd e = base;
+                if (c != 4)
+                    e += (a == b ? psi[g] : cd(0.0)) +
+                         lam * (in2 * p[g] + dinner[a] * dp[b][g] + dinner[b] * dp[a][g] + inner * d2p[g]);
+                herr = std::max(herr, std::abs(hs[c][g] - e));
+                href = std::max(href, std::abs(e - base));
+            }
+        }
+        std::printf("facade Hessian max err %.3e (scale %.3e)\n", herr, href);
+        if (herr > 1e-5 * href) return 9;
+    }
 #ifdef EFFM_USE_MODULE
     {
         // the reference's spelling, exported at global scope by the module
