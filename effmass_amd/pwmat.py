@@ -508,3 +508,146 @@ def load_cell(atom_config, vr_path, upf_by_z: dict, n=None):
     sp = [species_from_upf(read_upf(upf_by_z[z]), z) for z in zs]
     return synth.Cell(A=A, n=tuple(nvr), vloc=np.ascontiguousarray(data[0]), tau=frac,
                       species=species, species_list=sp)
+
+
+# Element symbols by atomic number (1-86), for matching atom.config's atomic
+# numbers to the element names in the UPF headers.
+ELEMENTS = ("H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca Sc Ti V Cr Mn Fe Co Ni Cu Zn Ga Ge As Se Br "
+            "Kr Rb Sr Y Zr Nb Mo Tc Ru Rh Pd Ag Cd In Sn Sb Te I Xe Cs Ba La Ce Pr Nd Pm Sm Eu Gd Tb Dy Ho "
+            "Er Tm Yb Lu Hf Ta W Re Os Ir Pt Au Hg Tl Pb Bi Po At Rn").split()
+
+
+class PWmatHamiltonian:
+    """The reference's file-based Hamiltonian interface
+    (src/H_psi/hamiltonian.cppm:115-160, hamiltonian.cpp:131-153): loadATOM /
+    loadNCPP / loadGKK / loadVR / loadEIGEN / loadWG with paths relative to
+    base_dir, finalize(), loadFromDirectory(), the has*() queries, then
+    at_k / gradient() / hessian() on the device Hamiltonian.  GKK k-point
+    ikpt is made resident on the device the first time an operator asks for
+    it.  NCPPs are matched to atom.config's atomic numbers by the element
+    name of their UPF header, as the reference's ncpp(atomic_number) does
+    (hamiltonian.cpp:199-206); finalize() raises if an atom has none."""
+
+    def __init__(self, base_dir=".", device: int = 0, hbm_budget: int = 0):
+        import os
+        self._os = os
+        self.base_dir = base_dir
+        self.device = device
+        self.hbm_budget = hbm_budget
+        self.atom = self.gkk = self.eigen = self.wg = None
+        self.vr_path = None
+        self.ncpps = {}                 # element name -> parsed UPF
+        self.H = None
+
+    def _p(self, path):
+        return path if self._os.path.isabs(path) else self._os.path.join(self.base_dir, path)
+
+    # -- step-by-step loading --------------------------------------------------
+    def loadATOM(self, path):
+        self.atom = read_atom_config(self._p(path))
+
+    def loadNCPP(self, path):
+        upf = read_upf(self._p(path))
+        self.ncpps[upf["element"]] = upf
+
+    def loadGKK(self, path):
+        self.gkk = GKKFile(self._p(path))
+
+    def loadVR(self, path):
+        read_vr(self._p(path))          # parsed now, so a bad file fails here
+        self.vr_path = self._p(path)
+
+    def loadEIGEN(self, path):
+        self.eigen = read_eigen(self._p(path))
+
+    def loadWG(self, path):
+        self.wg = WGFile(self._p(path))
+
+    def hasGKK(self):
+        return self.gkk is not None
+
+    def hasVR(self):
+        return self.vr_path is not None
+
+    def hasATOM(self):
+        return self.atom is not None
+
+    def hasEIGEN(self):
+        return self.eigen is not None
+
+    def hasWG(self):
+        return self.wg is not None
+
+    def finalize(self):
+        """Species tables (sortByL, diagonalizeNonlocal, BetaqTables) and
+        the device-resident cell; after every load, before any operator."""
+        if self.atom is None or self.gkk is None or self.vr_path is None:
+            raise RuntimeError("Hamiltonian: finalize() needs ATOM, GKK and VR")
+        _, Z, _ = self.atom
+        by_z = {}
+        for z in sorted(set(int(v) for v in Z)):
+            name = ELEMENTS[z - 1] if 1 <= z <= len(ELEMENTS) else str(z)
+            if name not in self.ncpps:
+                raise RuntimeError(f"Hamiltonian: no NCPP for element {name} (Z = {z})")
+            by_z[z] = self.ncpps[name]
+        A, Zs, frac = self.atom
+        nvr, _, data = read_vr(self.vr_path)
+        if tuple(nvr) != tuple(self.gkk.meta.n):
+            raise RuntimeError("VR grid does not match the GKK grid")
+        zs = sorted(by_z)
+        species = np.array([zs.index(int(z)) for z in Zs], dtype=np.int32)
+        cell = synth.Cell(A=A, n=tuple(nvr), vloc=np.ascontiguousarray(data[0]), tau=frac, species=species,
+                          species_list=[species_from_upf(by_z[z], z) for z in zs])
+        from .hamiltonian import Hamiltonian
+        if self.H is not None:
+            self.H.close()
+        self.H = Hamiltonian(cell, device=self.device, hbm_budget=self.hbm_budget)
+        return self
+
+    def loadFromDirectory(self):
+        """atom.config, the UPF files (UPF/ or base_dir), OUT.GKK, OUT.VR,
+        and OUT.EIGEN / OUT.WG when present; then finalize()."""
+        os = self._os
+        self.loadATOM("atom.config")
+        upf_dir = self._p("UPF") if os.path.isdir(self._p("UPF")) else self.base_dir
+        for f in sorted(os.listdir(upf_dir)):
+            if f.endswith((".UPF", ".upf")):
+                self.loadNCPP(os.path.join(upf_dir, f))
+        self.loadGKK("OUT.GKK")
+        self.loadVR("OUT.VR")
+        for name, load in (("OUT.EIGEN", self.loadEIGEN), ("OUT.WG", self.loadWG)):
+            if os.path.exists(self._p(name)):
+                load(name)
+        return self.finalize()
+
+    # -- operators -------------------------------------------------------------
+    def _resident(self, ikpt):
+        if self.H is None:
+            raise RuntimeError("Hamiltonian: call finalize() before at_k() / gradient() / hessian()")
+        if ikpt not in self.H._npw:
+            self.H.load_kpoint(ikpt, self.gkk.kpoint(ikpt))
+        return self.H
+
+    def at_k(self, ikpt: int):
+        return self._resident(ikpt).at_k(ikpt)
+
+    def gradient(self):
+        outer = self
+
+        class _G:
+            def at_k(self, ikpt):
+                return outer._resident(ikpt).gradient().at_k(ikpt)
+        return _G()
+
+    def hessian(self, delta: float = 1e-3):
+        outer = self
+
+        class _Hs:
+            def at_k(self, ikpt):
+                return outer._resident(ikpt).hessian(delta).at_k(ikpt)
+        return _Hs()
+
+    def close(self):
+        if self.H is not None:
+            self.H.close()
+            self.H = None

@@ -147,3 +147,34 @@ def test_hellmann_feynman_vs_eigen_fd():
                      max(worst[2], abs(h[d].imag))]
     log(f"max |HF - FD| {worst[0]:.2e}, max |cross| {worst[1]:.2e}, max |Im| {worst[2]:.2e} (Ha bohr)")
     assert worst[0] < TOL_FD_HF and worst[1] < TOL_ZERO and worst[2] < TOL_IMAG
+
+
+def test_file_loader_from_directory(tmp_path):
+    """loadFromDirectory() on the Si8 fixtures under the standard names, then
+    at_k / gradient() on GKK k-points made resident on first use: equal to
+    the oracle on the cell and k-points the readers build (1e-12)."""
+    from effmass_amd.hamiltonian import GradHPsi
+    d = tmp_path / "si8"
+    (d / "UPF").mkdir(parents=True)
+    for src, dst in (("si8_atom.config", "atom.config"), ("si8_OUT2.GKK", "OUT.GKK"), ("si8_IN.VR", "OUT.VR"),
+                     ("si8_OUT2.EIGEN", "OUT.EIGEN"), ("Si.SG15.PBE.UPF", "UPF/Si.SG15.PBE.UPF")):
+        os.symlink(os.path.join(G, src), d / dst)
+    h = pwmat.PWmatHamiltonian(str(d)).loadFromDirectory()
+    cell, g, _ = load("si8")
+    for ik in (0, 3):
+        kp = g.kpoint(ik)
+        psi = eigensolver.random_start(4, kp.npw, torch.from_numpy(kp.kin), seed=ik).numpy()
+        x = torch.from_numpy(psi).cuda()
+        hp = torch.zeros_like(x)
+        h.at_k(ik)(x, hp)
+        gx = [torch.zeros_like(x) for _ in range(3)]
+        h.gradient().at_k(ik)(x, GradHPsi(*gx))
+        torch.cuda.synchronize()
+        ref = oracle.hpsi(cell, kp, psi)
+        e = (np.linalg.norm(hp.cpu().numpy() - ref, axis=1) / np.linalg.norm(ref, axis=1)).max()
+        gref = oracle.dhdk(cell, kp, psi)
+        ge = max((np.linalg.norm(gx[a].cpu().numpy() - gref[a], axis=1) / np.linalg.norm(gref[a], axis=1)).max()
+                 for a in range(3))
+        log(f"file loader k {ik}: H {e:.2e}, dH/dk {ge:.2e}")
+        assert e <= 1e-12 and ge <= 1e-12
+    h.close()

@@ -242,3 +242,50 @@ def test_wg_spin_orbit_halves_and_round_trip(tmp_path):
         assert np.array_equal(w2.bands(ik), plain[ik])
     with pytest.raises(ValueError):
         w2.bands(0, spin="down")
+
+
+def _standard_dir(tmp_path, upf_subdir=True):
+    """The Si8 fixtures under the names loadFromDirectory reads."""
+    import os
+    d = tmp_path / "si8"
+    d.mkdir()
+    for src, dst in (("si8_atom.config", "atom.config"), ("si8_OUT2.GKK", "OUT.GKK"), ("si8_IN.VR", "OUT.VR"),
+                     ("si8_OUT2.EIGEN", "OUT.EIGEN")):
+        os.symlink(os.path.join(G, src), d / dst)
+    u = d / "UPF" if upf_subdir else d
+    u.mkdir(exist_ok=True)
+    os.symlink(os.path.join(G, "Si.SG15.PBE.UPF"), u / "Si.SG15.PBE.UPF")
+    return d
+
+
+def test_file_loader_steps_and_queries(tmp_path):
+    """The reference's step-by-step loading names (hamiltonian.cppm:121-160)
+    read the same data as the readers; has*() report what was loaded."""
+    d = _standard_dir(tmp_path)
+    h = pwmat.PWmatHamiltonian(str(d))
+    assert not (h.hasATOM() or h.hasGKK() or h.hasVR() or h.hasEIGEN() or h.hasWG())
+    h.loadATOM("atom.config")
+    h.loadNCPP("UPF/Si.SG15.PBE.UPF")
+    h.loadGKK("OUT.GKK")
+    h.loadVR("OUT.VR")
+    h.loadEIGEN("OUT.EIGEN")
+    assert h.hasATOM() and h.hasGKK() and h.hasVR() and h.hasEIGEN() and not h.hasWG()
+    assert list(h.ncpps) == ["Si"]
+    assert h.gkk.meta.n == (36, 36, 36)
+    A, Z, _ = h.atom
+    assert pwmat.ELEMENTS[int(Z[0]) - 1] == "Si"
+
+
+def test_file_loader_finalize_needs_every_element(tmp_path):
+    """finalize() refuses a cell with an atom whose element has no NCPP
+    (the reference's NCPPAtomCoverage check), before any device work."""
+    d = _standard_dir(tmp_path)
+    h = pwmat.PWmatHamiltonian(str(d))
+    h.loadATOM("atom.config")
+    h.loadGKK("OUT.GKK")
+    h.loadVR("OUT.VR")
+    with pytest.raises(RuntimeError, match="no NCPP for element Si"):
+        h.finalize()
+    h2 = pwmat.PWmatHamiltonian(str(d))
+    with pytest.raises(RuntimeError, match="needs ATOM, GKK and VR"):
+        h2.finalize()
